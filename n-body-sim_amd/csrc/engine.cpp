@@ -1,0 +1,641 @@
+// Host engine implementation (see engine.h).
+#include "engine.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "../../include/nbody.h"
+#include "kernels.h"
+
+namespace nbody {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  const int code = (e == hipErrorOutOfMemory) ? NBODY_E_NOMEM : NBODY_E_DEVICE;
+  throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+constexpr int64_t kAlign = 64;  // system ranges start on a wave boundary
+constexpr int kDoubleArrays = 25;
+constexpr int kFloatArrays = 7;
+
+int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+}  // namespace
+
+Engine::Engine(int device, hipStream_t stream, int mode, int64_t capacity,
+               int max_systems)
+    : device_(device),
+      stream_(stream),
+      mode_(mode),
+      cap_(round_up(std::max<int64_t>(capacity, 1), kAlign)),
+      max_sys_(std::max(max_systems, 1)) {
+  if (mode < MODE_F64_STRICT || mode > MODE_F32C)
+    throw Error(NBODY_E_INVALID, "unknown precision");
+  alloc_arena();
+  hip_check(hipMalloc((void**)&d_sys_, sizeof(SysDesc) * max_sys_),
+            "hipMalloc(system table)");
+  hip_check(hipMalloc((void**)&d_list_, sizeof(int32_t) * max_sys_),
+            "hipMalloc(system list)");
+  hip_check(hipMalloc((void**)&d_diag_, sizeof(double) * DIAG_COUNT * max_sys_),
+            "hipMalloc(diagnostics)");
+  h_sys_.assign(max_sys_, SysDesc{});
+  sys_.assign(max_sys_, Sys{});
+  free_[0] = cap_;
+}
+
+Engine::~Engine() {
+  drop_graphs();
+  hipStreamSynchronize(stream_);
+  hipFree(d_items_);
+  hipFree(d_diag_);
+  hipFree(d_list_);
+  hipFree(d_sys_);
+  hipFree(arena_mem_);
+}
+
+void Engine::alloc_arena() {
+  // Every array gets cap + kPad elements, each array 256-byte aligned.
+  const int64_t len = cap_ + kPad;
+  const size_t dbytes = round_up(len * 8, 256);
+  const size_t fbytes = round_up(len * 4, 256);
+  const size_t total = dbytes * kDoubleArrays + fbytes * kFloatArrays;
+  hip_check(hipMalloc(&arena_mem_, total), "hipMalloc(arena)");
+  // Zero everything once: slack and never-used ranges must hold finite values.
+  hip_check(hipMemsetAsync(arena_mem_, 0, total, stream_), "hipMemset(arena)");
+  char* p = (char*)arena_mem_;
+  auto D = [&]() {
+    double* r = (double*)p;
+    p += dbytes;
+    return r;
+  };
+  auto F = [&]() {
+    float* r = (float*)p;
+    p += fbytes;
+    return r;
+  };
+  A_.m = D();
+  A_.gm64 = D();
+  A_.x = D();
+  A_.y = D();
+  A_.z = D();
+  A_.vx = D();
+  A_.vy = D();
+  A_.vz = D();
+  for (int b = 0; b < 2; ++b) {
+    A_.sx[b] = D();
+    A_.sy[b] = D();
+    A_.sz[b] = D();
+  }
+  A_.ax = D();
+  A_.ay = D();
+  A_.az = D();
+  for (int q = 0; q < 9; ++q) A_.rk[q] = D();
+  A_.phi = D();
+  A_.gm32 = F();
+  for (int b = 0; b < 2; ++b) {
+    A_.fx[b] = F();
+    A_.fy[b] = F();
+    A_.fz[b] = F();
+  }
+  A_.cap = cap_;
+}
+
+void Engine::drop_graphs() {
+  for (auto& g : gexec_)
+    if (g) {
+      hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  gkey_ = -1;
+  gids_.clear();
+}
+
+void Engine::check_id(int id) const {
+  if (id < 0 || id >= max_sys_ || !sys_[id].live)
+    throw Error(NBODY_E_NOTFOUND, "unknown system id");
+}
+
+int Engine::nsystems() const {
+  int c = 0;
+  for (auto& s : sys_) c += s.live;
+  return c;
+}
+
+const SystemParams& Engine::params(int id) const {
+  check_id(id);
+  return sys_[id].p;
+}
+
+int64_t Engine::steps_done(int id) const {
+  check_id(id);
+  return sys_[id].steps;
+}
+
+bool Engine::fits(int64_t n) const {
+  const int64_t need = round_up(n, kAlign);
+  for (auto& f : free_)
+    if (f.second >= need) return true;
+  return false;
+}
+
+int64_t Engine::free_bodies() const {
+  int64_t t = 0;
+  for (auto& f : free_) t += f.second;
+  return t;
+}
+
+static void validate(const SystemParams& p) {
+  if (p.n <= 0) throw Error(NBODY_E_INVALID, "system must have n >= 1 bodies");
+  if (p.n > INT32_MAX) throw Error(NBODY_E_INVALID, "n too large");
+  if (p.integ < INTEG_EULER || p.integ > INTEG_RK4)
+    throw Error(NBODY_E_INVALID, "unknown integrator");
+  if (!std::isfinite(p.dt) || !std::isfinite(p.g) || !std::isfinite(p.eps2) ||
+      p.eps2 < 0.0)
+    throw Error(NBODY_E_INVALID, "dt, g and eps2 must be finite, eps2 >= 0");
+}
+
+int Engine::add(const SystemParams& p, const double* m, const double* x,
+                const double* y, const double* z, const double* vx,
+                const double* vy, const double* vz) {
+  validate(p);
+  if (!m || !x || !y || !z || !vx || !vy || !vz)
+    throw Error(NBODY_E_INVALID, "null body array");
+  const double* arrs[7] = {m, x, y, z, vx, vy, vz};
+  for (auto a : arrs)
+    for (int64_t i = 0; i < p.n; ++i)
+      if (!std::isfinite(a[i]))
+        throw Error(NBODY_E_INVALID, "non-finite body value");
+  int id = -1;
+  for (int k = 0; k < max_sys_; ++k)
+    if (!sys_[k].live) {
+      id = k;
+      break;
+    }
+  if (id < 0) throw Error(NBODY_E_NOSPACE, "system table full");
+  const int64_t need = round_up(p.n, kAlign);
+  auto it = free_.begin();
+  for (; it != free_.end(); ++it)
+    if (it->second >= need) break;
+  if (it == free_.end()) throw Error(NBODY_E_NOSPACE, "arena full");
+  const int64_t off = it->first, len = it->second;
+  free_.erase(it);
+  if (len > need) free_[off + need] = len - need;
+
+  Sys& s = sys_[id];
+  s = Sys{};
+  s.live = true;
+  s.p = p;
+  s.off = off;
+  s.span = need;
+  // centre of mass and its velocity, f64 in index order (origin of the f32
+  // snapshots; advanced by vcom * elapsed time at every re-centring)
+  double M = 0, c[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+  for (int64_t i = 0; i < p.n; ++i) {
+    M += m[i];
+    c[0] += m[i] * x[i];
+    c[1] += m[i] * y[i];
+    c[2] += m[i] * z[i];
+    v[0] += m[i] * vx[i];
+    v[1] += m[i] * vy[i];
+    v[2] += m[i] * vz[i];
+  }
+  for (int q = 0; q < 3; ++q) {
+    s.com[q] = (M != 0.0) ? c[q] / M : 0.0;
+    s.vcom[q] = (M != 0.0) ? v[q] / M : 0.0;
+  }
+  const size_t bytes = sizeof(double) * p.n;
+  double* dst[7] = {A_.m, A_.x, A_.y, A_.z, A_.vx, A_.vy, A_.vz};
+  for (int q = 0; q < 7; ++q)
+    hip_check(hipMemcpyAsync(dst[q] + off, arrs[q], bytes,
+                             hipMemcpyHostToDevice, stream_),
+              "hipMemcpy(upload)");
+  hip_check(hipMemsetAsync(A_.ax + off, 0, bytes, stream_), "hipMemset(acc)");
+  hip_check(hipMemsetAsync(A_.ay + off, 0, bytes, stream_), "hipMemset(acc)");
+  hip_check(hipMemsetAsync(A_.az + off, 0, bytes, stream_), "hipMemset(acc)");
+  // the caller's arrays may be released as soon as this returns
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  table_dirty_ = true;
+  drop_graphs();
+  return id;
+}
+
+void Engine::remove(int id) {
+  check_id(id);
+  Sys& s = sys_[id];
+  // Clear the range so stale values can never be read as sources.
+  for (double* a : {A_.x, A_.y, A_.z, A_.sx[0], A_.sy[0], A_.sz[0], A_.sx[1],
+                    A_.sy[1], A_.sz[1], A_.gm64})
+    hip_check(launch_fill_zero(a + s.off, s.span, stream_), "fill");
+  for (float* a : {A_.fx[0], A_.fy[0], A_.fz[0], A_.fx[1], A_.fy[1],
+                   A_.fz[1], A_.gm32})
+    hip_check(hipMemsetAsync(a + s.off, 0, sizeof(float) * s.span, stream_),
+              "hipMemset");
+  int64_t off = s.off, len = s.span;
+  auto nx = free_.lower_bound(off);
+  if (nx != free_.end() && nx->first == off + len) {
+    len += nx->second;
+    free_.erase(nx);
+  }
+  auto pv = free_.lower_bound(off);
+  if (pv != free_.begin()) {
+    --pv;
+    if (pv->first + pv->second == off) {
+      off = pv->first;
+      len += pv->second;
+      free_.erase(pv);
+    }
+  }
+  free_[off] = len;
+  s = Sys{};
+  table_dirty_ = true;
+  drop_graphs();
+}
+
+void Engine::set_params(int id, const SystemParams& p) {
+  check_id(id);
+  SystemParams q = p;
+  q.n = sys_[id].p.n;
+  validate(q);
+  Sys& s = sys_[id];
+  const bool g_changed = q.g != s.p.g;
+  if (q.integ != s.p.integ || q.dt != s.p.dt || q.eps2 != s.p.eps2 ||
+      g_changed) {
+    if (s.vv_primed) s.prepared = false;  // snapshot holds next positions
+    s.vv_primed = false;
+  }
+  if (g_changed) s.prepared = false;
+  s.p = q;
+  table_dirty_ = true;
+  drop_graphs();
+}
+
+void Engine::set_mode(int mode) {
+  if (mode < MODE_F64_STRICT || mode > MODE_F32C)
+    throw Error(NBODY_E_INVALID, "unknown precision");
+  if (mode == mode_) return;
+  mode_ = mode;
+  for (auto& s : sys_) {
+    s.vv_primed = false;
+    s.prepared = false;
+  }
+  drop_graphs();
+}
+
+void Engine::set_origin(Sys& s, SysDesc& d) const {
+  const double t = (double)s.steps * s.p.dt;
+  d.ox = s.com[0] + s.vcom[0] * t;
+  d.oy = s.com[1] + s.vcom[1] * t;
+  d.oz = s.com[2] + s.vcom[2] * t;
+}
+
+SysDesc Engine::desc_of(int id) const {
+  const Sys& s = sys_[id];
+  SysDesc d{};
+  d.off = s.off;
+  d.n = (int32_t)s.p.n;
+  d.integ = s.p.integ;
+  d.dt = s.p.dt;
+  d.eps2 = s.p.eps2;
+  d.g = s.p.g;
+  d.eps2f = (float)s.p.eps2;
+  return d;
+}
+
+void Engine::upload_table() {
+  if (!table_dirty_) return;
+  for (int k = 0; k < max_sys_; ++k) {
+    if (!sys_[k].live) continue;
+    SysDesc d = desc_of(k);
+    d.ox = h_sys_[k].ox;
+    d.oy = h_sys_[k].oy;
+    d.oz = h_sys_[k].oz;
+    h_sys_[k] = d;
+  }
+  hip_check(hipMemcpyAsync(d_sys_, h_sys_.data(), sizeof(SysDesc) * max_sys_,
+                           hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(system table)");
+  // the host copy must stay unchanged until the copy has been consumed
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  table_dirty_ = false;
+}
+
+void Engine::ensure_items(size_t n) {
+  if (n <= items_cap_) return;
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  if (d_items_) hipFree(d_items_);
+  d_items_ = nullptr;
+  items_cap_ = std::max<size_t>(n, 2 * items_cap_);
+  hip_check(hipMalloc((void**)&d_items_, sizeof(WorkItem) * items_cap_),
+            "hipMalloc(work list)");
+}
+
+void Engine::tiles_of(int id, int stage, int src,
+                      std::vector<WorkItem>& out) const {
+  const int64_t tiles = (sys_[id].p.n + kWave - 1) / kWave;
+  for (int64_t t = 0; t < tiles; ++t)
+    out.push_back(WorkItem{id, (int32_t)t, stage, src});
+}
+
+void Engine::run_round(std::vector<WorkItem>& items, bool potential) {
+  if (items.empty()) return;
+  sort_items(items);
+  ensure_items(items.size());
+  hip_check(hipMemcpyAsync(d_items_, items.data(),
+                           sizeof(WorkItem) * items.size(),
+                           hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(work list)");
+  hipError_t e = potential
+                     ? launch_potential(A_, d_items_, (int)items.size(),
+                                        d_sys_, stream_)
+                     : launch_force(mode_, A_, d_items_, (int)items.size(),
+                                    d_sys_, stream_);
+  hip_check(e, "force launch");
+  // the pageable host list must not change before the copy is consumed
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+// Rebuilds gm and the current snapshot of the listed systems that need it:
+// never prepared, or due for re-centring.  A system is re-centred when its
+// own step count is a multiple of its rebase interval; the new origin is the
+// centre of mass advanced to the system's current time (com + vcom * t).
+// The schedule depends only on the system itself, so a system's results do
+// not depend on which other systems share its launches.
+void Engine::prepare(const std::vector<int>& ids) {
+  std::vector<WorkItem> items;
+  for (int id : ids) {
+    Sys& s = sys_[id];
+    const bool due = s.rebase_every > 0 && (s.steps % s.rebase_every) == 0;
+    if (s.prepared && !due) continue;
+    SysDesc d = desc_of(id);
+    set_origin(s, d);
+    const bool moved = d.ox != h_sys_[id].ox || d.oy != h_sys_[id].oy ||
+                       d.oz != h_sys_[id].oz;
+    if (s.prepared && !(moved && mode_ == MODE_F32C)) continue;
+    if (moved) table_dirty_ = true;
+    h_sys_[id].ox = d.ox;
+    h_sys_[id].oy = d.oy;
+    h_sys_[id].oz = d.oz;
+    const int stage = (s.p.integ == INTEG_VV && s.vv_primed) ? STAGE_VV_PRIME
+                                                             : STAGE_ACC;
+    tiles_of(id, stage, s.cur, items);
+    s.prepared = true;
+  }
+  upload_table();
+  if (items.empty()) return;
+  ensure_items(items.size());
+  hip_check(hipMemcpyAsync(d_items_, items.data(),
+                           sizeof(WorkItem) * items.size(),
+                           hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(work list)");
+  hip_check(launch_prepare(A_, d_items_, (int)items.size(), d_sys_, stream_),
+            "prepare launch");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+void Engine::set_rebase_interval(int id, int64_t every) {
+  check_id(id);
+  sys_[id].rebase_every = every;
+}
+
+// Work lists of one step for the listed systems: lists[r] for rounds 0..3,
+// with each system's snapshot parity as it will be when that round runs,
+// given `flip` extra steps already taken by Euler / velocity-Verlet systems
+// (RK4 systems return to the same buffer after their four stages).
+void Engine::step_lists(const std::vector<int>& ids, int flip,
+                        std::vector<WorkItem> (&lists)[4]) const {
+  for (int r = 0; r < 4; ++r) {
+    lists[r].clear();
+    for (int id : ids) {
+      const Sys& s = sys_[id];
+      if (s.p.integ == INTEG_RK4) {
+        tiles_of(id, STAGE_RK0 + r, s.cur ^ (r & 1), lists[r]);
+      } else if (r == 0) {
+        tiles_of(id, s.p.integ == INTEG_EULER ? STAGE_EULER : STAGE_VV,
+                 s.cur ^ flip, lists[r]);
+      }
+    }
+    sort_items(lists[r]);
+  }
+}
+
+void Engine::sort_items(std::vector<WorkItem>& items) const {
+  // longest systems first: the dispatcher then packs the short tail last
+  std::stable_sort(items.begin(), items.end(),
+                   [&](const WorkItem& a, const WorkItem& b) {
+                     return sys_[a.sys].p.n > sys_[b.sys].p.n;
+                   });
+}
+
+// Primes velocity-Verlet systems that carry no acceleration yet.
+void Engine::prime(const std::vector<int>& ids) {
+  std::vector<WorkItem> items;
+  for (int id : ids) {
+    const Sys& s = sys_[id];
+    if (s.p.integ == INTEG_VV && !s.vv_primed)
+      tiles_of(id, STAGE_VV_PRIME, s.cur, items);
+  }
+  if (items.empty()) return;
+  run_round(items);
+  for (int id : ids) {
+    Sys& s = sys_[id];
+    if (s.p.integ == INTEG_VV && !s.vv_primed) {
+      s.vv_primed = true;
+      s.cur ^= 1;
+    }
+  }
+}
+
+// Uploads the step-A and step-B lists (A: current parity, B: after one
+// step) into one device buffer; returns the device offsets and counts.
+void Engine::upload_step_lists(const std::vector<int>& ids, int offs[8],
+                               int cnts[8]) {
+  std::vector<WorkItem> la[4], lb[4];
+  step_lists(ids, 0, la);
+  step_lists(ids, 1, lb);
+  std::vector<WorkItem> all;
+  for (int r = 0; r < 8; ++r) {
+    const std::vector<WorkItem>& l = r < 4 ? la[r] : lb[r - 4];
+    offs[r] = (int)all.size();
+    cnts[r] = (int)l.size();
+    all.insert(all.end(), l.begin(), l.end());
+  }
+  ensure_items(all.size());
+  if (!all.empty()) {
+    hip_check(hipMemcpyAsync(d_items_, all.data(), sizeof(WorkItem) * all.size(),
+                             hipMemcpyHostToDevice, stream_),
+              "hipMemcpy(work lists)");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  }
+}
+
+void Engine::launch_step(const int offs[8], const int cnts[8], int which) {
+  for (int r = 0; r < 4; ++r) {
+    const int k = which * 4 + r;
+    if (cnts[k] == 0) continue;
+    hip_check(launch_force(mode_, A_, d_items_ + offs[k], cnts[k], d_sys_,
+                           stream_),
+              "force launch");
+  }
+}
+
+void Engine::finish_steps(const std::vector<int>& ids, int64_t nsteps) {
+  for (int id : ids) {
+    Sys& s = sys_[id];
+    if (s.p.integ != INTEG_RK4 && (nsteps & 1)) s.cur ^= 1;
+    s.steps += nsteps;
+  }
+}
+
+void Engine::step(const std::vector<int>& ids, int64_t nsteps) {
+  if (nsteps < 0) throw Error(NBODY_E_INVALID, "nsteps < 0");
+  for (int id : ids) check_id(id);
+  if (nsteps == 0 || ids.empty()) return;
+  prepare(ids);
+  prime(ids);
+  int offs[8], cnts[8];
+  upload_step_lists(ids, offs, cnts);
+  for (int64_t t = 0; t < nsteps; ++t) launch_step(offs, cnts, (int)(t & 1));
+  finish_steps(ids, nsteps);
+}
+
+void Engine::accelerations(int id, double* ax, double* ay, double* az) {
+  check_id(id);
+  Sys& s = sys_[id];
+  if (s.vv_primed) {  // the snapshot holds next-step positions: rebuild
+    s.vv_primed = false;
+    s.prepared = false;
+  }
+  prepare({id});
+  std::vector<WorkItem> items;
+  tiles_of(id, STAGE_ACC, s.cur, items);
+  run_round(items);
+  s.vv_primed = false;  // the carried acceleration was overwritten
+  if (ax || ay || az) {
+    const size_t bytes = sizeof(double) * s.p.n;
+    if (ax)
+      hip_check(hipMemcpyAsync(ax, A_.ax + s.off, bytes, hipMemcpyDeviceToHost,
+                               stream_),
+                "hipMemcpy(acc)");
+    if (ay)
+      hip_check(hipMemcpyAsync(ay, A_.ay + s.off, bytes, hipMemcpyDeviceToHost,
+                               stream_),
+                "hipMemcpy(acc)");
+    if (az)
+      hip_check(hipMemcpyAsync(az, A_.az + s.off, bytes, hipMemcpyDeviceToHost,
+                               stream_),
+                "hipMemcpy(acc)");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  }
+}
+
+void Engine::diagnostics(const std::vector<int>& ids, double* out) {
+  for (int id : ids) check_id(id);
+  if (ids.empty()) return;
+  // the potential pass reads the f64 snapshot of the current positions
+  const int saved = mode_;
+  std::vector<WorkItem> items;
+  std::vector<int32_t> list(ids.begin(), ids.end());
+  // f64 snapshot of the master positions into the spare buffer
+  std::vector<WorkItem> prep;
+  for (int id : ids) tiles_of(id, STAGE_ACC, sys_[id].cur ^ 1, prep);
+  upload_table();
+  ensure_items(prep.size());
+  hip_check(hipMemcpyAsync(d_items_, prep.data(), sizeof(WorkItem) * prep.size(),
+                           hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(work list)");
+  hip_check(launch_prepare(A_, d_items_, (int)prep.size(), d_sys_, stream_),
+            "prepare launch");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  for (int id : ids) tiles_of(id, STAGE_POT, sys_[id].cur ^ 1, items);
+  run_round(items, /*potential=*/true);
+  (void)saved;
+  hip_check(hipMemcpyAsync(d_list_, list.data(), sizeof(int32_t) * list.size(),
+                           hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(list)");
+  hip_check(launch_diag(A_, d_sys_, d_list_, (int)list.size(), d_diag_,
+                        stream_),
+            "diag launch");
+  hip_check(hipMemcpyAsync(out, d_diag_, sizeof(double) * DIAG_COUNT * ids.size(),
+                           hipMemcpyDeviceToHost, stream_),
+            "hipMemcpy(diag)");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+void Engine::download(int id, double* m, double* x, double* y, double* z,
+                      double* vx, double* vy, double* vz) {
+  check_id(id);
+  const Sys& s = sys_[id];
+  const size_t bytes = sizeof(double) * s.p.n;
+  double* dst[7] = {m, x, y, z, vx, vy, vz};
+  const double* src[7] = {A_.m, A_.x, A_.y, A_.z, A_.vx, A_.vy, A_.vz};
+  for (int q = 0; q < 7; ++q)
+    if (dst[q])
+      hip_check(hipMemcpyAsync(dst[q], src[q] + s.off, bytes,
+                               hipMemcpyDeviceToHost, stream_),
+                "hipMemcpy(download)");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+void Engine::download_positions_async(int id, double* x, double* y, double* z,
+                                      hipStream_t s) {
+  check_id(id);
+  const Sys& sy = sys_[id];
+  const size_t bytes = sizeof(double) * sy.p.n;
+  hip_check(hipMemcpyAsync(x, A_.x + sy.off, bytes, hipMemcpyDeviceToHost, s),
+            "hipMemcpy(snapshot)");
+  hip_check(hipMemcpyAsync(y, A_.y + sy.off, bytes, hipMemcpyDeviceToHost, s),
+            "hipMemcpy(snapshot)");
+  hip_check(hipMemcpyAsync(z, A_.z + sy.off, bytes, hipMemcpyDeviceToHost, s),
+            "hipMemcpy(snapshot)");
+}
+
+void Engine::synchronize() {
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+// The same loop as step(), with the launches of two consecutive steps
+// (lists A then B) captured once into a HIP graph and replayed.  The graph
+// only references the device work-list buffer and counts, so it stays valid
+// while the composition (and therefore every count) is unchanged; the list
+// contents are re-uploaded on every call.
+void Engine::step_graph(const std::vector<int>& ids, int64_t nsteps) {
+  if (nsteps < 0) throw Error(NBODY_E_INVALID, "nsteps < 0");
+  for (int id : ids) check_id(id);
+  if (nsteps == 0 || ids.empty()) return;
+  prepare(ids);
+  prime(ids);
+  int offs[8], cnts[8];
+  upload_step_lists(ids, offs, cnts);
+  int64_t key = (int64_t)(uintptr_t)d_items_ * 31 + mode_;
+  for (int k = 0; k < 8; ++k) key = key * 1000003 + cnts[k];
+  if (key != gkey_ || ids != gids_ || !gexec_[0]) {
+    drop_graphs();
+    hipGraph_t g = nullptr;
+    hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
+              "hipStreamBeginCapture");
+    try {
+      launch_step(offs, cnts, 0);
+      launch_step(offs, cnts, 1);
+    } catch (...) {
+      hipStreamEndCapture(stream_, &g);
+      if (g) hipGraphDestroy(g);
+      throw;
+    }
+    hip_check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
+    hipError_t e = hipGraphInstantiate(&gexec_[0], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    hip_check(e, "hipGraphInstantiate");
+    gkey_ = key;
+    gids_ = ids;
+  }
+  for (int64_t t = 0; t + 1 < nsteps; t += 2)
+    hip_check(hipGraphLaunch(gexec_[0], stream_), "hipGraphLaunch");
+  if (nsteps & 1) launch_step(offs, cnts, 0);
+  finish_steps(ids, nsteps);
+}
+
+}  // namespace nbody

@@ -1,0 +1,144 @@
+// Host engine: one device arena holding any number of independent systems,
+// stepped together in shared launches.
+//
+// The single-system C API (a DeviceBodies counterpart), the batched API and
+// the request server are all built on this one class.  A step of the engine
+// is up to four force rounds: round 0 carries every listed system (the only
+// evaluation of Euler and carried-acceleration velocity Verlet, RK4's first
+// stage); rounds 1-3 carry only RK4 systems.  Each round is one launch over a
+// work list of (system, 64-target tile) items, longest systems first.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "nbody_types.h"
+
+namespace nbody {
+
+// Error carrying a public NBODY_E_* code.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+
+struct SystemParams {
+  int64_t n = 0;
+  int integ = INTEG_EULER;
+  double dt = 0.0, g = 1.0, eps2 = 0.0;
+};
+
+class Engine {
+ public:
+  // capacity: bodies the arena can hold; max_systems: table size.
+  Engine(int device, hipStream_t stream, int mode, int64_t capacity,
+         int max_systems);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  int mode() const { return mode_; }
+  // Changing the arithmetic rebuilds every snapshot and drops carried state.
+  void set_mode(int mode);
+
+  // Uploads a system; returns its id.  Throws Error(NBODY_E_NOSPACE) when no
+  // contiguous range of the arena is free.
+  int add(const SystemParams& p, const double* m, const double* x,
+          const double* y, const double* z, const double* vx,
+          const double* vy, const double* vz);
+  void remove(int id);
+  bool fits(int64_t n) const;
+  int64_t free_bodies() const;
+  int64_t capacity() const { return cap_; }
+  int nsystems() const;
+  const SystemParams& params(int id) const;
+  int64_t steps_done(int id) const;
+  void set_params(int id, const SystemParams& p);
+  // Re-centre the f32 origin of a system whenever its step count is a
+  // multiple of `every` (default 1: at every step call).
+  void set_rebase_interval(int id, int64_t every);
+
+  // Advances every listed system by nsteps of its own integrator.
+  void step(const std::vector<int>& ids, int64_t nsteps);
+  // F(x) of one system into the device acceleration arrays (and optionally
+  // to the host).  Drops a velocity-Verlet carry of that system.
+  void accelerations(int id, double* ax, double* ay, double* az);
+  // Per-system diagnostics, DIAG_COUNT doubles per listed system.
+  void diagnostics(const std::vector<int>& ids, double* out);
+  // Downloads the master state (any pointer may be null).  Synchronous.
+  void download(int id, double* m, double* x, double* y, double* z,
+                double* vx, double* vy, double* vz);
+  // Asynchronous copy of positions into caller-owned (pinned) host memory.
+  void download_positions_async(int id, double* x, double* y, double* z,
+                                hipStream_t s);
+  void synchronize();
+
+  // Graph-captured multi-step loop for a fixed set of systems (batched API).
+  void step_graph(const std::vector<int>& ids, int64_t nsteps);
+
+  const Arena& arena() const { return A_; }
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  struct Sys {
+    bool live = false;
+    SystemParams p;
+    int64_t off = 0, span = 0;  // arena range (span: rounded allocation)
+    int cur = 0;                // snapshot buffer holding the next sources
+    bool vv_primed = false;
+    bool prepared = false;
+    int64_t steps = 0;
+    int64_t rebase_every = 1;
+    double com[3] = {0, 0, 0};  // centre of mass at upload
+    double vcom[3] = {0, 0, 0};  // centre-of-mass velocity
+  };
+
+  void alloc_arena();
+  void upload_table();
+  void ensure_items(size_t n);
+  void prepare(const std::vector<int>& ids);
+  void set_origin(Sys& s, SysDesc& d) const;
+  SysDesc desc_of(int id) const;
+  void tiles_of(int id, int stage, int src, std::vector<WorkItem>& out) const;
+  void run_round(std::vector<WorkItem>& items, bool potential = false);
+  void sort_items(std::vector<WorkItem>& items) const;
+  void step_lists(const std::vector<int>& ids, int flip,
+                  std::vector<WorkItem> (&lists)[4]) const;
+  void prime(const std::vector<int>& ids);
+  void upload_step_lists(const std::vector<int>& ids, int offs[8],
+                         int cnts[8]);
+  void launch_step(const int offs[8], const int cnts[8], int which);
+  void finish_steps(const std::vector<int>& ids, int64_t nsteps);
+  void check_id(int id) const;
+
+  int device_;
+  hipStream_t stream_;
+  int mode_;
+  int64_t cap_;
+  int max_sys_;
+  Arena A_{};
+  void* arena_mem_ = nullptr;
+  SysDesc* d_sys_ = nullptr;
+  std::vector<SysDesc> h_sys_;
+  WorkItem* d_items_ = nullptr;
+  size_t items_cap_ = 0;
+  int32_t* d_list_ = nullptr;
+  double* d_diag_ = nullptr;
+  std::vector<Sys> sys_;
+  std::map<int64_t, int64_t> free_;  // offset -> length
+  bool table_dirty_ = true;
+
+  // graph cache for step_graph
+  hipGraphExec_t gexec_[1] = {nullptr};
+  std::vector<int> gids_;
+  int64_t gkey_ = -1;
+  void drop_graphs();
+};
+
+}  // namespace nbody

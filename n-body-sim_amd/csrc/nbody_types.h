@@ -1,0 +1,86 @@
+// Types shared by the host engine and the gfx950 kernels.
+//
+// Memory layout in HBM (one arena per engine, structure of arrays, f64 state):
+//   m, gm64 (= G*m), gm32         bodies' masses, G folded in once per system
+//   x, y, z, vx, vy, vz           master state, binary64
+//   sx/sy/sz[2]                   f64 source snapshot, double-buffered
+//   fx/fy/fz[2]                   f32 source snapshot (re-centred), double-buffered
+//   ax, ay, az                    carried acceleration (velocity Verlet) or the
+//                                 output of a plain force evaluation
+//   rk[9]                         RK4 running sums and next-stage velocity
+//   phi                           per-body potential (energy diagnostics)
+// Systems occupy disjoint [off, off+n) ranges of every array.  Every array has
+// kPad elements of zeroed slack past the arena end so a source chunk that
+// straddles the last body stays in bounds.
+#pragma once
+#include <stdint.h>
+
+namespace nbody {
+
+constexpr int kWave = 64;        // gfx950 wavefront
+constexpr int kBlock = 256;      // 4 waves per workgroup, one work item each
+constexpr int kPad = 128;        // slack elements after the arena
+
+enum Mode : int32_t {            // force arithmetic (public NBODY_PRECISION_*)
+  MODE_F64_STRICT = 0,           // reference operation order, no contraction,
+                                 // correctly rounded sqrt and divide
+  MODE_F64 = 1,                  // f64, rsq + refinement, fused multiply-add
+  MODE_F32C = 2,                 // f32 pair terms on re-centred snapshots,
+                                 // compensated accumulation
+};
+
+enum Integ : int32_t { INTEG_EULER = 0, INTEG_VV = 1, INTEG_RK4 = 2 };
+
+// What the epilogue of one force evaluation does for a system.
+enum Stage : int32_t {
+  STAGE_ACC = 0,       // store a into ax/ay/az; state unchanged
+  STAGE_EULER = 1,     // v += a*dt; x += v*dt; snapshot <- x
+  STAGE_VV_PRIME = 2,  // carry a; snapshot <- x + ((v*dt) + ((0.5*a)*dt2))
+  STAGE_VV = 3,        // finish step (x, v), carry a, snapshot <- next x
+  STAGE_RK0 = 4,       // RK4 stages 1..4
+  STAGE_RK1 = 5,
+  STAGE_RK2 = 6,
+  STAGE_RK3 = 7,
+  STAGE_POT = 8,       // store potential sum_j G m_j / sqrt(r2 + eps2) in phi
+};
+
+struct SysDesc {       // one per resident system (80 B)
+  int64_t off;         // first body in the arena
+  int32_t n;           // body count
+  int32_t integ;       // Integ
+  double dt;
+  double eps2;         // softening squared, f64
+  double g;            // gravitational constant
+  double ox, oy, oz;   // origin of the f32 snapshot (f64)
+  float eps2f;         // eps2 rounded to f32
+  int32_t pad_[3];
+};
+static_assert(sizeof(SysDesc) == 80, "SysDesc layout");
+
+struct WorkItem {      // one wave: 64 consecutive targets of one system
+  int32_t sys;         // index into the SysDesc table
+  int32_t tile;        // targets [off + 64*tile, off + 64*tile + 64) clipped to n
+  int32_t stage;       // Stage
+  int32_t src;         // snapshot buffer read (0/1); the epilogue writes src^1
+};
+static_assert(sizeof(WorkItem) == 16, "WorkItem layout");
+
+struct Arena {         // device pointers, passed by value to every kernel
+  double *m, *gm64;
+  float *gm32;
+  double *x, *y, *z, *vx, *vy, *vz;
+  double *sx[2], *sy[2], *sz[2];
+  float *fx[2], *fy[2], *fz[2];
+  double *ax, *ay, *az;
+  double *rk[9];
+  double *phi;
+  int64_t cap;         // bodies (excluding the kPad slack)
+};
+
+// Per-system diagnostics written by the reduction kernel (9 doubles).
+enum Diag : int32_t {
+  DIAG_K = 0, DIAG_U = 1, DIAG_PX = 2, DIAG_PY = 3, DIAG_PZ = 4,
+  DIAG_MX = 5, DIAG_MY = 6, DIAG_MZ = 7, DIAG_M = 8, DIAG_COUNT = 9
+};
+
+}  // namespace nbody

@@ -1,0 +1,434 @@
+// Served form: a request loop with continuous batching.
+//
+// One worker thread owns the engine.  Requests wait in a FIFO until their
+// device footprint (a contiguous arena range plus a table slot) is free;
+// admission is first-fit in arrival order, and a request that does not fit
+// may be bypassed by later ones at most kMaxBypass times before admission
+// stops behind it (no starvation).  All resident systems advance together in
+// epochs; an epoch ends when some system reaches a snapshot boundary (a
+// multiple of snapshot_every of its own steps) or its last step, so every
+// system is snapshotted, re-centred and retired at the same own-step counts
+// whether it runs alone or in a mix.  New requests join at the next epoch.
+// Snapshots are copied out after the epoch and queued as events; when the
+// queued snapshot bytes exceed the configured cap the worker pauses stepping
+// until the client drains events (backpressure).
+#include "server.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "engine.h"
+
+namespace nbody {
+
+namespace {
+constexpr int kMaxBypass = 64;
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+  uint64_t id = 0;
+  SystemParams p;
+  int64_t steps = 0;
+  bool want_snapshots = false;
+  std::vector<double> m, x, y, z, vx, vy, vz;
+  double t_submit = 0;
+  int bypassed = 0;
+};
+
+struct Active {
+  Request req;
+  int sys = -1;
+  double t_admit = 0;
+  bool cancel = false;
+};
+
+struct Event {
+  nbody_event ev{};
+  std::vector<double> x, y, z, vx, vy, vz;
+  size_t bytes() const {
+    return sizeof(double) * (x.size() + y.size() + z.size() + vx.size() +
+                             vy.size() + vz.size());
+  }
+};
+
+int mode_of(int precision) {
+  switch (precision) {
+    case NBODY_PRECISION_F64_STRICT:
+      return MODE_F64_STRICT;
+    case NBODY_PRECISION_F64:
+      return MODE_F64;
+    case NBODY_PRECISION_F32C:
+      return MODE_F32C;
+  }
+  throw Error(NBODY_E_INVALID, "unknown precision");
+}
+
+int64_t evals_per_step(int integ) { return integ == INTEG_RK4 ? 4 : 1; }
+}  // namespace
+
+class Server {
+ public:
+  explicit Server(const nbody_server_config& cfg) : cfg_(cfg) {
+    if (cfg.capacity_bodies <= 0 || cfg.max_systems <= 0 ||
+        cfg.snapshot_every <= 0)
+      throw Error(NBODY_E_INVALID,
+                  "capacity_bodies, max_systems and snapshot_every must be > 0");
+    mode_ = mode_of(cfg.precision);
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+      throw Error(NBODY_E_DEVICE, "no GPU available");
+    if (cfg.device < 0 || cfg.device >= count)
+      throw Error(NBODY_E_INVALID, "device index out of range");
+    hip_check(hipSetDevice(cfg.device), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking),
+              "hipStreamCreate");
+    try {
+      eng_ = std::make_unique<Engine>(cfg.device, stream_, mode_,
+                                      cfg.capacity_bodies, cfg.max_systems);
+    } catch (...) {
+      hipStreamDestroy(stream_);
+      throw;
+    }
+    worker_ = std::thread([this] { run(); });
+  }
+
+  ~Server() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_work_.notify_all();
+    cv_events_.notify_all();
+    if (worker_.joinable()) worker_.join();
+    hipSetDevice(cfg_.device);
+    eng_.reset();
+    hipStreamDestroy(stream_);
+  }
+
+  uint64_t submit(const nbody_request& r) {
+    if (r.n <= 0) throw Error(NBODY_E_INVALID, "request needs n >= 1");
+    if (!(r.dt > 0) || !std::isfinite(r.dt))
+      throw Error(NBODY_E_INVALID, "dt must be finite and > 0");
+    if (r.steps < 0) throw Error(NBODY_E_INVALID, "steps < 0");
+    if (r.integrator < NBODY_INTEGRATOR_EULER ||
+        r.integrator > NBODY_INTEGRATOR_RUNGE_KUTTA)
+      throw Error(NBODY_E_INVALID, "unknown integrator");
+    if (!r.masses || !r.pos_x || !r.pos_y || !r.pos_z || !r.vel_x ||
+        !r.vel_y || !r.vel_z)
+      throw Error(NBODY_E_INVALID, "null body array");
+    if ((r.n + 63) / 64 * 64 > eng_->capacity())
+      throw Error(NBODY_E_NOSPACE, "request larger than the server arena");
+    Request q;
+    q.p.n = r.n;
+    q.p.integ = r.integrator;  // NBODY_INTEGRATOR_* == Integ values
+    q.p.dt = r.dt;
+    q.p.g = r.g;
+    q.p.eps2 = r.eps2;
+    q.steps = r.steps;
+    q.want_snapshots = r.want_snapshots != 0;
+    auto cp = [&](const double* a, std::vector<double>& v) {
+      v.assign(a, a + r.n);
+      for (double d : v)
+        if (!std::isfinite(d))
+          throw Error(NBODY_E_INVALID, "non-finite body value");
+    };
+    cp(r.masses, q.m);
+    cp(r.pos_x, q.x);
+    cp(r.pos_y, q.y);
+    cp(r.pos_z, q.z);
+    cp(r.vel_x, q.vx);
+    cp(r.vel_y, q.vy);
+    cp(r.vel_z, q.vz);
+    if (!std::isfinite(r.g) || !std::isfinite(r.eps2) || r.eps2 < 0)
+      throw Error(NBODY_E_INVALID, "g and eps2 must be finite, eps2 >= 0");
+    q.t_submit = now_s();
+    uint64_t id;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      id = q.id = ++next_id_;
+      queue_.push_back(std::move(q));
+      ++submitted_;
+    }
+    cv_work_.notify_all();
+    return id;
+  }
+
+  void poll(int timeout_ms, nbody_event* ev, double* x, double* y, double* z,
+            double* vx, double* vy, double* vz, int64_t capacity) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto ready = [&] { return !events_.empty() || stop_; };
+    if (timeout_ms < 0)
+      cv_events_.wait(lk, ready);
+    else
+      cv_events_.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+    if (events_.empty()) throw Error(NBODY_E_TIMEOUT, "no event");
+    Event& e = events_.front();
+    *ev = e.ev;
+    const bool has_data = !e.x.empty();
+    if (has_data && (x || y || z || vx || vy || vz) && capacity < e.ev.n)
+      throw Error(NBODY_E_NOSPACE, "output arrays smaller than event n");
+    auto put = [&](double* dst, const std::vector<double>& v) {
+      if (dst && !v.empty()) std::memcpy(dst, v.data(), sizeof(double) * v.size());
+    };
+    put(x, e.x);
+    put(y, e.y);
+    put(z, e.z);
+    put(vx, e.vx);
+    put(vy, e.vy);
+    put(vz, e.vz);
+    queued_bytes_ -= e.bytes();
+    events_.pop_front();
+    lk.unlock();
+    cv_work_.notify_all();  // may release backpressure
+  }
+
+  void cancel(uint64_t id) {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto it = queue_.begin(); it != queue_.end(); ++it)
+      if (it->id == id) {
+        Event e;
+        fill(e.ev, NBODY_EVENT_CANCELLED, *it, 0, 0.0, "cancelled while queued");
+        push_event(std::move(e));
+        queue_.erase(it);
+        return;
+      }
+    auto a = active_.find(id);
+    if (a == active_.end()) throw Error(NBODY_E_NOTFOUND, "unknown request id");
+    a->second.cancel = true;
+  }
+
+  void stats(double* out) {
+    std::lock_guard<std::mutex> lk(mu_);
+    out[0] = (double)submitted_;
+    out[1] = (double)admitted_;
+    out[2] = (double)completed_;
+    out[3] = (double)active_.size();
+    out[4] = (double)queue_.size();
+    out[5] = (double)epoch_steps_;
+    out[6] = interactions_;
+    out[7] = (double)failed_;
+  }
+
+ private:
+  static void fill(nbody_event& ev, int kind, const Request& q, int64_t step,
+                   double t_admit, const char* msg) {
+    ev.kind = kind;
+    ev.request_id = q.id;
+    ev.step = step;
+    ev.n = q.p.n;
+    ev.t_submit = q.t_submit;
+    ev.t_admit = t_admit;
+    ev.t_event = now_s();
+    std::strncpy(ev.message, msg ? msg : "", sizeof(ev.message) - 1);
+    ev.message[sizeof(ev.message) - 1] = '\0';
+  }
+
+  void push_event(Event&& e) {  // mu_ held
+    queued_bytes_ += e.bytes();
+    events_.push_back(std::move(e));
+    cv_events_.notify_all();
+  }
+
+  // Admits queued requests that fit (worker thread, mu_ held).
+  std::vector<Request> take_admissible() {
+    std::vector<Request> out;
+    int64_t free_b = eng_->free_bodies();
+    int free_slots = cfg_.max_systems - eng_->nsystems();
+    bool blocked = false;
+    for (auto it = queue_.begin(); it != queue_.end() && free_slots > 0;) {
+      const int64_t need = (it->p.n + 63) / 64 * 64;
+      // contiguity is checked against the arena; pending admissions are
+      // accounted by the running free-body count
+      if (!blocked && need <= free_b && eng_->fits(it->p.n)) {
+        free_b -= need;
+        --free_slots;
+        out.push_back(std::move(*it));
+        it = queue_.erase(it);
+        continue;
+      }
+      if (++it->bypassed > kMaxBypass) blocked = true;
+      if (blocked) break;
+      ++it;
+    }
+    return out;
+  }
+
+  void run() {
+    hipSetDevice(cfg_.device);
+    for (;;) {
+      std::vector<Request> admit;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_work_.wait(lk, [&] {
+          if (stop_) return true;
+          const bool backpressured =
+              cfg_.max_snapshot_bytes > 0 &&
+              queued_bytes_ > (size_t)cfg_.max_snapshot_bytes;
+          if (backpressured) return false;
+          return !active_.empty() || !queue_.empty();
+        });
+        if (stop_) return;
+        admit = take_admissible();
+      }
+      // upload admitted requests (engine is owned by this thread)
+      for (auto& q : admit) {
+        Active a;
+        a.t_admit = now_s();
+        try {
+          a.sys = eng_->add(q.p, q.m.data(), q.x.data(), q.y.data(),
+                            q.z.data(), q.vx.data(), q.vy.data(), q.vz.data());
+          eng_->set_rebase_interval(a.sys, cfg_.snapshot_every);
+        } catch (const std::exception& ex) {
+          std::lock_guard<std::mutex> lk(mu_);
+          Event e;
+          fill(e.ev, NBODY_EVENT_FAILED, q, 0, a.t_admit, ex.what());
+          push_event(std::move(e));
+          ++failed_;
+          continue;
+        }
+        // free the host copy of the initial state
+        for (auto* v : {&q.m, &q.x, &q.y, &q.z, &q.vx, &q.vy, &q.vz})
+          std::vector<double>().swap(*v);
+        a.req = std::move(q);
+        std::lock_guard<std::mutex> lk(mu_);
+        ++admitted_;
+        const uint64_t id = a.req.id;
+        active_.emplace(id, std::move(a));
+      }
+      // retire cancelled and zero-step requests, choose the epoch length
+      std::vector<int> ids;
+      int64_t k = cfg_.snapshot_every;
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto it = active_.begin(); it != active_.end();) {
+          Active& a = it->second;
+          const int64_t done = eng_->steps_done(a.sys);
+          if (a.cancel || done >= a.req.steps) {
+            retire(a, a.cancel ? NBODY_EVENT_CANCELLED : NBODY_EVENT_DONE);
+            it = active_.erase(it);
+            continue;
+          }
+          ids.push_back(a.sys);
+          k = std::min(k, cfg_.snapshot_every - done % cfg_.snapshot_every);
+          k = std::min(k, a.req.steps - done);
+          ++it;
+        }
+      }
+      if (ids.empty()) continue;
+      std::sort(ids.begin(), ids.end());
+      double inter = 0;
+      try {
+        eng_->step_graph(ids, k);
+        eng_->synchronize();
+      } catch (const std::exception& ex) {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto it = active_.begin(); it != active_.end();) {
+          retire(it->second, NBODY_EVENT_FAILED, ex.what());
+          ++failed_;
+          it = active_.erase(it);
+        }
+        continue;
+      }
+      // snapshots at own-step multiples of snapshot_every, and completions
+      std::lock_guard<std::mutex> lk(mu_);
+      epoch_steps_ += k;
+      for (auto it = active_.begin(); it != active_.end();) {
+        Active& a = it->second;
+        const double n = (double)a.req.p.n;
+        inter += n * (n - 1.0) * (double)(k * evals_per_step(a.req.p.integ));
+        const int64_t done = eng_->steps_done(a.sys);
+        if (done >= a.req.steps) {
+          retire(a, NBODY_EVENT_DONE);
+          it = active_.erase(it);
+          continue;
+        }
+        if (a.req.want_snapshots && done % cfg_.snapshot_every == 0) {
+          Event e;
+          fill(e.ev, NBODY_EVENT_SNAPSHOT, a.req, done, a.t_admit, "");
+          e.x.resize(a.req.p.n);
+          e.y.resize(a.req.p.n);
+          e.z.resize(a.req.p.n);
+          eng_->download(a.sys, nullptr, e.x.data(), e.y.data(), e.z.data(),
+                         nullptr, nullptr, nullptr);
+          push_event(std::move(e));
+        }
+        ++it;
+      }
+      interactions_ += inter;
+    }
+  }
+
+  // Emits the final event of an active request and frees its slot (mu_ held).
+  void retire(Active& a, int kind, const char* msg = "") {
+    Event e;
+    const int64_t done = eng_->steps_done(a.sys);
+    fill(e.ev, kind, a.req, done, a.t_admit, msg);
+    if (kind == NBODY_EVENT_DONE) {
+      const int64_t n = a.req.p.n;
+      for (auto* v : {&e.x, &e.y, &e.z, &e.vx, &e.vy, &e.vz}) v->resize(n);
+      try {
+        eng_->download(a.sys, nullptr, e.x.data(), e.y.data(), e.z.data(),
+                       e.vx.data(), e.vy.data(), e.vz.data());
+        ++completed_;
+      } catch (const std::exception& ex) {
+        fill(e.ev, NBODY_EVENT_FAILED, a.req, done, a.t_admit, ex.what());
+        for (auto* v : {&e.x, &e.y, &e.z, &e.vx, &e.vy, &e.vz}) v->clear();
+        ++failed_;
+      }
+    }
+    try {
+      eng_->remove(a.sys);
+    } catch (...) {
+    }
+    push_event(std::move(e));
+  }
+
+  nbody_server_config cfg_;
+  int mode_ = MODE_F32C;
+  hipStream_t stream_ = nullptr;
+  std::unique_ptr<Engine> eng_;
+  std::thread worker_;
+  std::mutex mu_;
+  std::condition_variable cv_work_, cv_events_;
+  std::deque<Request> queue_;
+  std::map<uint64_t, Active> active_;
+  std::deque<Event> events_;
+  size_t queued_bytes_ = 0;
+  bool stop_ = false;
+  uint64_t next_id_ = 0;
+  int64_t submitted_ = 0, admitted_ = 0, completed_ = 0, failed_ = 0;
+  int64_t epoch_steps_ = 0;
+  double interactions_ = 0;
+};
+
+Server* server_create(const nbody_server_config& cfg) {
+  return new Server(cfg);
+}
+uint64_t server_submit(Server* s, const nbody_request& req) {
+  return s->submit(req);
+}
+void server_poll(Server* s, int timeout_ms, nbody_event* ev, double* x,
+                 double* y, double* z, double* vx, double* vy, double* vz,
+                 int64_t capacity) {
+  s->poll(timeout_ms, ev, x, y, z, vx, vy, vz, capacity);
+}
+void server_cancel(Server* s, uint64_t id) { s->cancel(id); }
+void server_stats(Server* s, double* out) { s->stats(out); }
+void server_destroy(Server* s) { delete s; }
+
+}  // namespace nbody

@@ -1,0 +1,535 @@
+"""Python host layer over ``libnbody_amd.so`` (C ABI in ``include/nbody.h``).
+
+Mirrors the reference's GPU selection and trait surface so callers read the
+same way as the Rust code they replace:
+
+* ``GravityMethod`` / ``IntegratorMethod`` with ``from_str`` and
+  ``gpu_create`` -- src/args.rs:84-187
+* ``CudaManager``-like context ``GpuContext`` -- src/gpu/cuda_manager.rs:12-62
+* ``DeviceBodies`` -- src/gpu/device_bodies.rs:7-55
+* ``GpuNewtonParallelGravity.calculate_accelerations`` -- src/gravity.rs:30-36,
+  src/gravity/gpu/newton_parallel.rs:20-46
+* ``GpuEulerIntegrator`` / ``GpuVelocityVerletIntegrator`` /
+  ``GpuRungeKuttaIntegrator.step`` -- src/integrators/gpu/gpu_integrator.rs:8-15
+* ``GpuSimulation.step`` / ``snapshot`` -- src/simulation/gpu_simulation.rs:45-65
+
+There is no CPU fallback: every compute call goes through the HIP library and
+raises ``NbodyError`` when it is missing or no GPU is present.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "libnbody_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_PKG_DIR), "include", "nbody.h")
+
+OK = 0
+E_INVALID, E_UNSUPPORTED, E_DEVICE, E_NOMEM = -1, -2, -3, -4
+E_NOSPACE, E_NOTFOUND, E_STATE, E_TIMEOUT = -5, -6, -7, -8
+
+EVENT_SNAPSHOT, EVENT_DONE, EVENT_FAILED, EVENT_CANCELLED = 1, 2, 3, 4
+
+
+class NbodyError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+def _bind(lib):
+    i, i64, u64, d = ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double
+    vp = ctypes.c_void_p
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "nbody_last_error": ([], ctypes.c_char_p),
+        "nbody_version": ([], ctypes.c_char_p),
+        "nbody_ctx_create": ([i, pp], i),
+        "nbody_ctx_destroy": ([vp], i),
+        "nbody_ctx_synchronize": ([vp], i),
+        "nbody_ctx_device_info": ([vp, ctypes.c_char_p, i, ctypes.POINTER(i)], i),
+        "nbody_ctx_record": ([vp, i], i),
+        "nbody_ctx_elapsed": ([vp, i, i, ctypes.POINTER(ctypes.c_float)], i),
+        "nbody_bodies_create": ([vp, i64] + [_D] * 7 + [pp], i),
+        "nbody_bodies_download": ([vp, vp] + [_D] * 7, i),
+        "nbody_bodies_destroy": ([vp], i),
+        "nbody_bodies_len": ([vp], i64),
+        "nbody_calculate_accelerations": ([vp, vp, i, d, d, i, _D, _D, _D], i),
+        "nbody_step": ([vp, vp, i, i, i, d, d, d, i64], i),
+        "nbody_energy_momentum": ([vp, vp, d, d, _D], i),
+        "nbody_scenario_plummer": ([i64, u64, d, d, d] + [_D] * 7, i),
+        "nbody_scenario_star_system": ([i64, u64, d, d, d, d, d, d, d] + [_D] * 7, i),
+        "nbody_scenario_colliding": ([i64, u64, d, d, d] + [_D] * 7, i),
+        "nbody_batch_create": ([vp, i, ctypes.POINTER(i64), ctypes.POINTER(i), _D, _D,
+                                _D, i] + [_D] * 7 + [pp], i),
+        "nbody_batch_step": ([vp, vp, i64], i),
+        "nbody_batch_download": ([vp, vp] + [_D] * 6, i),
+        "nbody_batch_energy": ([vp, vp, _D], i),
+        "nbody_batch_destroy": ([vp], i),
+        "nbody_server_create": ([ctypes.POINTER(ServerConfig), pp], i),
+        "nbody_server_submit": ([vp, ctypes.POINTER(Request), ctypes.POINTER(u64)], i),
+        "nbody_server_poll": ([vp, i, ctypes.POINTER(Event)] + [_D] * 6 + [i64], i),
+        "nbody_server_cancel": ([vp, u64], i),
+        "nbody_server_stats": ([vp, _D], i),
+        "nbody_server_destroy": ([vp], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("precision", ctypes.c_int),
+                ("capacity_bodies", ctypes.c_int64), ("max_systems", ctypes.c_int),
+                ("snapshot_every", ctypes.c_int), ("max_snapshot_bytes", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("integrator", ctypes.c_int),
+                ("gravity_method", ctypes.c_int), ("g", ctypes.c_double),
+                ("dt", ctypes.c_double), ("eps2", ctypes.c_double),
+                ("steps", ctypes.c_int64), ("want_snapshots", ctypes.c_int)] + [
+        (f, _D) for f in ("masses", "pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z")]
+
+
+class Event(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("request_id", ctypes.c_uint64),
+                ("step", ctypes.c_int64), ("n", ctypes.c_int64),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_event", ctypes.c_double), ("message", ctypes.c_char * 128)]
+
+
+def load_library():
+    """Load the built HIP library (raises NbodyError if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NbodyError(E_DEVICE, f"{LIB_PATH} is not built; run "
+                             "__graft_entry__.build() (make -C n-body-sim_amd/csrc)")
+        _lib = _bind(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def _check(rc: int):
+    if rc != OK:
+        msg = load_library().nbody_last_error().decode(errors="replace")
+        raise NbodyError(rc, msg)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---- reference selection enums (src/args.rs:84-187) -----------------------
+class Precision(enum.IntEnum):
+    F64_STRICT = 0
+    F64 = 1
+    F32C = 2
+
+
+class GravityMethod(enum.IntEnum):
+    NEWTON = 0
+    NEWTON_PARALLEL = 1
+    BARNES_HUT = 2
+
+    @classmethod
+    def from_str(cls, s: str) -> "GravityMethod":
+        # src/args.rs:125-136
+        table = {"newton": cls.NEWTON, "newton-parallel": cls.NEWTON_PARALLEL,
+                 "barnes-hut": cls.BARNES_HUT}
+        try:
+            return table[s.lower()]
+        except KeyError:
+            raise ValueError(f"Invalid gravity method: {s}") from None
+
+    def gpu_create(self, params: "SimulationParameters",
+                   precision: Precision = Precision.F64) -> "GpuNewtonParallelGravity":
+        # src/args.rs:111-123: only newton-parallel runs on the GPU; eps2 = eps^2
+        if self is not GravityMethod.NEWTON_PARALLEL:
+            raise NbodyError(E_UNSUPPORTED, "GPU-accelerated version of "
+                             f"{self.name} gravity method is not implemented")
+        return GpuNewtonParallelGravity(params.g_constant,
+                                        params.softening_factor * params.softening_factor,
+                                        precision)
+
+
+class IntegratorMethod(enum.IntEnum):
+    EULER = 0
+    VELOCITY_VERLET = 1
+    RUNGE_KUTTA = 2
+
+    @classmethod
+    def from_str(cls, s: str) -> "IntegratorMethod":
+        # src/args.rs:176-187
+        table = {"euler": cls.EULER, "velocity-verlet": cls.VELOCITY_VERLET,
+                 "runge-kutta": cls.RUNGE_KUTTA}
+        try:
+            return table[s.lower()]
+        except KeyError:
+            raise ValueError(f"Invalid integrator method: {s}") from None
+
+    def gpu_create(self, time_step: float) -> "GpuIntegrator":
+        # src/args.rs:165-174 (Euler only there; all three here)
+        return {IntegratorMethod.EULER: GpuEulerIntegrator,
+                IntegratorMethod.VELOCITY_VERLET: GpuVelocityVerletIntegrator,
+                IntegratorMethod.RUNGE_KUTTA: GpuRungeKuttaIntegrator}[self](time_step)
+
+
+@dataclass
+class SimulationParameters:
+    """src/args.rs SimulationParameters (the fields the hot path reads)."""
+    g_constant: float = 1.0
+    time_step: float = 0.01
+    softening_factor: float = 0.005
+
+
+@dataclass
+class Bodies:
+    """SoA f64 bodies, src/body.rs:52-61."""
+    masses: np.ndarray
+    pos_x: np.ndarray
+    pos_y: np.ndarray
+    pos_z: np.ndarray
+    vel_x: np.ndarray
+    vel_y: np.ndarray
+    vel_z: np.ndarray
+
+    def __post_init__(self):
+        for f in self.fields():
+            setattr(self, f, _f64(getattr(self, f)))
+        n = len(self.masses)
+        if any(len(getattr(self, f)) != n for f in self.fields()):
+            raise ValueError("all body arrays must have the same length")  # body.rs:74-82
+
+    @staticmethod
+    def fields():
+        return ("masses", "pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z")
+
+    def __len__(self):
+        return len(self.masses)
+
+    def arrays(self):
+        return [getattr(self, f) for f in self.fields()]
+
+    def copy(self) -> "Bodies":
+        return Bodies(*[a.copy() for a in self.arrays()])
+
+
+class GpuContext:
+    """The CudaManager counterpart: device, stream, init check."""
+
+    def __init__(self, device: int = 0):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib.nbody_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        _check(load_library().nbody_ctx_synchronize(self._h))
+
+    def device_info(self):
+        buf = ctypes.create_string_buffer(128)
+        cu = ctypes.c_int()
+        _check(load_library().nbody_ctx_device_info(self._h, buf, 128, ctypes.byref(cu)))
+        return buf.value.decode(), cu.value
+
+    def record(self, slot: int):
+        """Record timing event `slot` (0..7) on the context's stream."""
+        _check(load_library().nbody_ctx_record(self._h, slot))
+
+    def elapsed_ms(self, slot_from: int, slot_to: int) -> float:
+        ms = ctypes.c_float()
+        _check(load_library().nbody_ctx_elapsed(self._h, slot_from, slot_to,
+                                                ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().nbody_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBodies:
+    """src/gpu/device_bodies.rs:7-55: device-resident SoA bodies."""
+
+    def __init__(self, ctx: GpuContext, bodies: Bodies):
+        lib = load_library()
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(lib.nbody_bodies_create(ctx.handle, len(bodies),
+                                       *[_p(a) for a in bodies.arrays()],
+                                       ctypes.byref(h)))
+        self._h = h
+        self._n = len(bodies)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return self._n
+
+    def download(self) -> Bodies:
+        arrs = [np.empty(self._n) for _ in range(7)]
+        _check(load_library().nbody_bodies_download(self.ctx.handle, self._h,
+                                                    *[_p(a) for a in arrs]))
+        return Bodies(*arrs)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().nbody_bodies_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuNewtonParallelGravity:
+    """src/gravity/gpu/newton_parallel.rs: direct sum on the GPU."""
+
+    def __init__(self, g_constant: float, eps2: float,
+                 precision: Precision = Precision.F64):
+        self.g_constant = float(g_constant)
+        self.eps2 = float(eps2)
+        self.precision = Precision(precision)
+
+    def calculate_accelerations(self, gpu: GpuContext, bodies: DeviceBodies,
+                                download: bool = True):
+        """Accelerations into the device arrays; returned as (ax, ay, az)."""
+        n = len(bodies)
+        out = [np.empty(n) for _ in range(3)] if download else [None] * 3
+        _check(load_library().nbody_calculate_accelerations(
+            gpu.handle, bodies.handle, int(GravityMethod.NEWTON_PARALLEL),
+            self.g_constant, self.eps2, int(self.precision), *[_p(a) for a in out]))
+        return tuple(out) if download else None
+
+
+class GpuIntegrator:
+    integrator: IntegratorMethod
+
+    def __init__(self, dt: float):
+        self.dt = float(dt)
+
+    def step(self, gpu: GpuContext, bodies: DeviceBodies,
+             gravity: GpuNewtonParallelGravity, nsteps: int = 1):
+        """GpuIntegrator::step, nsteps times in one call."""
+        _check(load_library().nbody_step(
+            gpu.handle, bodies.handle, int(self.integrator),
+            int(GravityMethod.NEWTON_PARALLEL), int(gravity.precision), self.dt,
+            gravity.g_constant, gravity.eps2, int(nsteps)))
+
+
+class GpuEulerIntegrator(GpuIntegrator):
+    integrator = IntegratorMethod.EULER
+
+
+class GpuVelocityVerletIntegrator(GpuIntegrator):
+    integrator = IntegratorMethod.VELOCITY_VERLET
+
+
+class GpuRungeKuttaIntegrator(GpuIntegrator):
+    integrator = IntegratorMethod.RUNGE_KUTTA
+
+
+class GpuSimulation:
+    """src/simulation/gpu_simulation.rs: owns context, bodies and methods."""
+
+    def __init__(self, bodies: Bodies, params: SimulationParameters,
+                 gravity: GravityMethod = GravityMethod.NEWTON_PARALLEL,
+                 integrator: IntegratorMethod = IntegratorMethod.EULER,
+                 precision: Precision = Precision.F64, device: int = 0):
+        self.gpu = GpuContext(device)
+        self.gravity = gravity.gpu_create(params, precision)
+        self.integrator = integrator.gpu_create(params.time_step)
+        self.bodies = DeviceBodies(self.gpu, bodies)
+        self.time = 0.0
+        self.dt = params.time_step
+
+    def step(self, nsteps: int = 1):
+        self.integrator.step(self.gpu, self.bodies, self.gravity, nsteps)
+        for _ in range(nsteps):
+            self.time += self.dt  # accumulated as at cpu_simulation.rs:37
+
+    def snapshot(self) -> Bodies:
+        return self.bodies.download()
+
+    def energy_momentum(self):
+        out = np.zeros(10)
+        _check(load_library().nbody_energy_momentum(
+            self.gpu.handle, self.bodies.handle, self.gravity.g_constant,
+            self.gravity.eps2, _p(out)))
+        return out
+
+
+# ---- scenarios ------------------------------------------------------------
+def _scenario(fn, n, *args) -> Bodies:
+    arrs = [np.empty(n) for _ in range(7)]
+    _check(fn(n, *args, *[_p(a) for a in arrs]))
+    return Bodies(*arrs)
+
+
+def plummer(n: int, seed: int, a: float = 1.0, total_mass: float = 1.0,
+            g: float = 1.0) -> Bodies:
+    return _scenario(load_library().nbody_scenario_plummer, n, seed, a, total_mass, g)
+
+
+def star_system(n: int, seed: int, central_mass: float = 1.0, m_lo: float = 1e-7,
+                m_hi: float = 1e-4, r_lo: float = 0.5, r_hi: float = 10.0,
+                max_incl_deg: float = 5.0, g: float = 1.0) -> Bodies:
+    return _scenario(load_library().nbody_scenario_star_system, n, seed,
+                     central_mass, m_lo, m_hi, r_lo, r_hi, max_incl_deg, g)
+
+
+def colliding(n: int, seed: int, separation: float = 10.0, rel_speed: float = 0.5,
+              g: float = 1.0) -> Bodies:
+    return _scenario(load_library().nbody_scenario_colliding, n, seed,
+                     separation, rel_speed, g)
+
+
+# ---- batched form -----------------------------------------------------------
+class Batch:
+    """Many independent systems in one arena, stepped by graph-captured launches."""
+
+    def __init__(self, ctx: GpuContext, systems, integrators, dts, gs, eps2s,
+                 precision: Precision = Precision.F32C):
+        lib = load_library()
+        self.ctx = ctx
+        self.n = np.array([len(b) for b in systems], dtype=np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.n)[:-1]]).astype(np.int64)
+        packed = [np.concatenate([getattr(b, f) for b in systems]) for f in Bodies.fields()]
+        integ = np.ascontiguousarray(integrators, dtype=np.int32)
+        dts, gs, eps2s = (_f64(np.broadcast_to(v, len(systems))) for v in (dts, gs, eps2s))
+        h = ctypes.c_void_p()
+        _check(lib.nbody_batch_create(
+            ctx.handle, len(systems), self.n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            integ.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(dts), _p(gs), _p(eps2s),
+            int(precision), *[_p(a) for a in packed], ctypes.byref(h)))
+        self._h = h
+        self.total = int(self.n.sum())
+
+    def step(self, nsteps: int):
+        _check(load_library().nbody_batch_step(self.ctx.handle, self._h, int(nsteps)))
+
+    def download(self):
+        arrs = [np.empty(self.total) for _ in range(6)]
+        _check(load_library().nbody_batch_download(self.ctx.handle, self._h,
+                                                   *[_p(a) for a in arrs]))
+        return arrs
+
+    def energy(self) -> np.ndarray:
+        out = np.zeros((len(self.n), 10))
+        _check(load_library().nbody_batch_energy(self.ctx.handle, self._h, _p(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().nbody_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- served form ------------------------------------------------------------
+class Server:
+    """Continuous-batching request server (thread-safe)."""
+
+    def __init__(self, capacity_bodies: int, max_systems: int = 1024,
+                 snapshot_every: int = 10, precision: Precision = Precision.F32C,
+                 device: int = 0, max_snapshot_bytes: int = 1 << 30):
+        lib = load_library()
+        cfg = ServerConfig(device, int(precision), capacity_bodies, max_systems,
+                           snapshot_every, max_snapshot_bytes)
+        h = ctypes.c_void_p()
+        _check(lib.nbody_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, bodies: Bodies, integrator: IntegratorMethod, dt: float,
+               steps: int, g: float = 1.0, eps2: float = 0.0,
+               want_snapshots: bool = False,
+               gravity: GravityMethod = GravityMethod.NEWTON_PARALLEL) -> int:
+        arrs = bodies.arrays()
+        req = Request(len(bodies), int(integrator), int(gravity), g, dt, eps2, steps,
+                      int(want_snapshots), *[_p(a) for a in arrs])
+        rid = ctypes.c_uint64()
+        _check(load_library().nbody_server_submit(self._h, ctypes.byref(req),
+                                                  ctypes.byref(rid)))
+        return rid.value
+
+    def poll(self, timeout_ms: int = -1, capacity: int = 0):
+        """Returns (event dict, arrays) or None on timeout."""
+        lib = load_library()
+        ev = Event()
+        while True:
+            arrs = [np.empty(max(capacity, 1)) for _ in range(6)]
+            rc = lib.nbody_server_poll(self._h, timeout_ms, ctypes.byref(ev),
+                                       *[_p(a) for a in arrs], capacity)
+            if rc == E_TIMEOUT:
+                return None
+            if rc == E_NOSPACE:
+                capacity = int(ev.n)
+                continue
+            _check(rc)
+            break
+        n = int(ev.n)
+        data = None
+        if ev.kind == EVENT_SNAPSHOT:
+            data = [a[:n].copy() for a in arrs[:3]]
+        elif ev.kind == EVENT_DONE:
+            data = [a[:n].copy() for a in arrs]
+        return ({"kind": ev.kind, "id": ev.request_id, "step": ev.step, "n": n,
+                 "t_submit": ev.t_submit, "t_admit": ev.t_admit, "t_event": ev.t_event,
+                 "message": ev.message.decode(errors="replace")}, data)
+
+    def cancel(self, request_id: int):
+        _check(load_library().nbody_server_cancel(self._h, request_id))
+
+    def stats(self) -> dict:
+        out = np.zeros(8)
+        _check(load_library().nbody_server_stats(self._h, _p(out)))
+        keys = ("submitted", "admitted", "completed", "resident", "queued",
+                "shared_steps", "interactions", "failed")
+        return dict(zip(keys, out.tolist()))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().nbody_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,153 @@
+"""CPU oracle for the direct-sum gravity + integrator path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
+CPU-baseline leg of bench.py -- as the checker, never as the thing measured
+or shipped.  The product (the HIP library) never routes through it.
+
+ctypes bindings to ``oracle/liboracle.so`` (built from ``oracle/oracle.c`` by
+``oracle/Makefile``), a binary64 restatement of the reference's CPU path:
+
+* ``newton``            -- src/gravity/cpu/newton.rs:69-141
+* ``newton_parallel``   -- src/gravity/cpu/newton_parallel.rs:31-53
+* ``newton_targets``    -- the same per-target sum for a sampled target list
+* ``integrate``         -- src/integrators/cpu/{euler.rs:57-82,
+                           velocity_verlet.rs:63-104, runge_kutta.rs:149-281}
+* ``energy``            -- softened K and U (no reference counterpart)
+
+The reference is Rust and cannot be built in this environment (no cargo or
+rustc), so there is no ``oracle/_ref`` build; the restatement is pinned by
+closed forms, brute force, invariants and an independent numpy evaluation in
+tests/test_oracle.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+EULER, VELOCITY_VERLET, RUNGE_KUTTA = 0, 1, 2
+INTEGRATORS = {"euler": EULER, "velocity-verlet": VELOCITY_VERLET,
+               "runge-kutta": RUNGE_KUTTA}
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    i64, d, i = ctypes.c_int64, ctypes.c_double, ctypes.c_int
+    lib.oracle_newton.argtypes = [i64, d, d, _D, _D, _D, _D, _D, _D, _D]
+    lib.oracle_newton.restype = None
+    lib.oracle_newton_parallel.argtypes = [i64, d, d, _D, _D, _D, _D, _D, _D,
+                                           _D, i]
+    lib.oracle_newton_parallel.restype = None
+    lib.oracle_newton_targets.argtypes = [i64, d, d, _D, _D, _D, _D, _I64,
+                                          i64, _D, _D, _D, i]
+    lib.oracle_newton_targets.restype = None
+    lib.oracle_integrate.argtypes = [i, i64, d, d, d, i64, _D, _D, _D, _D, _D,
+                                     _D, _D, i]
+    lib.oracle_integrate.restype = ctypes.c_int
+    lib.oracle_energy.argtypes = [i64, d, d, _D, _D, _D, _D, _D, _D, _D, _D]
+    lib.oracle_energy.restype = None
+    _lib = lib
+    return lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(_D)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def default_threads() -> int:
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:  # pragma: no cover
+        return os.cpu_count() or 1
+
+
+def newton(m, x, y, z, g: float, softening: float):
+    """Serial f64 accelerations (newton.rs:69-87). Returns (ax, ay, az)."""
+    lib = _load()
+    m, x, y, z = map(_f64, (m, x, y, z))
+    n = m.shape[0]
+    ax, ay, az = (np.empty(n) for _ in range(3))
+    lib.oracle_newton(n, g, softening, _p(m), _p(x), _p(y), _p(z), _p(ax),
+                      _p(ay), _p(az))
+    return ax, ay, az
+
+
+def newton_parallel(m, x, y, z, g: float, softening: float,
+                    nthreads: int | None = None):
+    """Threaded f64 accelerations (newton_parallel.rs:31-53)."""
+    lib = _load()
+    m, x, y, z = map(_f64, (m, x, y, z))
+    n = m.shape[0]
+    ax, ay, az = (np.empty(n) for _ in range(3))
+    lib.oracle_newton_parallel(n, g, softening, _p(m), _p(x), _p(y), _p(z),
+                               _p(ax), _p(ay), _p(az),
+                               nthreads or default_threads())
+    return ax, ay, az
+
+
+def newton_targets(m, x, y, z, g: float, softening: float, idx,
+                   nthreads: int | None = None):
+    """f64 accelerations of the targets ``idx`` only (same arithmetic)."""
+    lib = _load()
+    m, x, y, z = map(_f64, (m, x, y, z))
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    k = idx.shape[0]
+    ax, ay, az = (np.empty(k) for _ in range(3))
+    lib.oracle_newton_targets(m.shape[0], g, softening, _p(m), _p(x), _p(y),
+                              _p(z), idx.ctypes.data_as(_I64), k, _p(ax),
+                              _p(ay), _p(az), nthreads or default_threads())
+    return ax, ay, az
+
+
+def integrate(integrator, m, x, y, z, vx, vy, vz, g: float, softening: float,
+              dt: float, nsteps: int, nthreads: int | None = None):
+    """Advance copies of the state by ``nsteps`` reference steps.
+
+    ``integrator`` is 0/1/2 or "euler"/"velocity-verlet"/"runge-kutta".
+    Returns (x, y, z, vx, vy, vz) as new arrays.
+    """
+    lib = _load()
+    if isinstance(integrator, str):
+        integrator = INTEGRATORS[integrator]
+    m = _f64(m)
+    st = [np.array(a, dtype=np.float64, copy=True) for a in (x, y, z, vx, vy, vz)]
+    rc = lib.oracle_integrate(int(integrator), m.shape[0], g, softening, dt,
+                              int(nsteps), _p(m), *[_p(a) for a in st],
+                              nthreads or default_threads())
+    if rc != 0:
+        raise ValueError("oracle_integrate failed")
+    return tuple(st)
+
+
+def energy(m, x, y, z, vx, vy, vz, g: float, softening: float):
+    """Softened (K, U) in f64."""
+    lib = _load()
+    arrs = list(map(_f64, (m, x, y, z, vx, vy, vz)))
+    out = np.zeros(2)
+    lib.oracle_energy(arrs[0].shape[0], g, softening, *[_p(a) for a in arrs],
+                      _p(out))
+    return float(out[0]), float(out[1])

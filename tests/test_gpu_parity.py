@@ -1,0 +1,227 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Tolerances (written here, from the path's contract):
+* NBODY_PRECISION_F64_STRICT -- bit-identical accelerations and trajectories.
+* NBODY_PRECISION_F64        -- max relative acceleration error <= 1e-12.
+* NBODY_PRECISION_F32C       -- max relative acceleration error <= 1e-5.
+Relative error per body is |a - a_ref|_2 / |a_ref|_2.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import nbody_amd as nb
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+EXAMPLES = {  # data/examples/*/initial_conditions.csv and config.json values
+    "binary-system": ([[10.0, -2.0, 0.0, 0.0, 0.0, -1.118, 0.0],
+                       [10.0, 2.0, 0.0, 0.0, 0.0, 1.118, 0.0]], 1.0, 0.025, 0.05),
+    "figure-eight": ([[1.0, -0.97000436, 0.24308753, 0.1, 0.466203685, 0.43236573, 0.0],
+                      [1.0, 0.97000436, -0.24308753, 0.1, 0.466203685, 0.43236573, 0.0],
+                      [1.0, 0.0, 0.0, 0.1, -0.93240737, -0.86473146, 0.0]], 1.0, 0.01, 0.005),
+    "spin-merge": ([[1.0, 0.0, 0.0, 0.0, 0.2, 0.0, 0.0],
+                    [1.0, 0.0, 0.0, 1.5, 0.0, 0.2, 0.0]], 1.0, 0.01, 0.05),
+    "sun-earth-moon": ([[1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0],
+                        [3.003e-06, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0],
+                        [3.694e-08, 1.00257, 0.0, 0.0, 0.0, 1.0342, 0.0]], 1.0, 0.001, 0.0),
+}
+INTEG = [nb.IntegratorMethod.EULER, nb.IntegratorMethod.VELOCITY_VERLET,
+         nb.IntegratorMethod.RUNGE_KUTTA]
+
+
+def example(name):
+    rows, g, dt, eps = EXAMPLES[name]
+    a = np.array(rows, dtype=np.float64).T
+    return nb.Bodies(*a), g, dt, eps
+
+
+def rel_err(a, ref):
+    a = np.stack(a, 1)
+    ref = np.stack(ref, 1)
+    return np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+
+
+def gpu_acc(gpu, b, g, eps, prec):
+    db = nb.DeviceBodies(gpu, b)
+    grav = nb.GpuNewtonParallelGravity(g, eps * eps, prec)
+    out = grav.calculate_accelerations(gpu, db)
+    db.close()
+    return out
+
+
+def test_context(gpu):
+    name, cus = gpu.device_info()
+    print("device", name, cus, flush=True)
+    assert name.startswith("gfx950")
+    assert cus > 0
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 63, 64, 65, 130, 1000, 1024])
+def test_strict_accelerations_bit_exact(gpu, n):
+    b = nb.plummer(n, 100 + n)
+    ref = oracle.newton(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, 0.01)
+    got = gpu_acc(gpu, b, 1.0, 0.01, nb.Precision.F64_STRICT)
+    for q in range(3):
+        assert np.array_equal(got[q], ref[q]), f"component {q}"
+
+
+@pytest.mark.parametrize("name", sorted(EXAMPLES))
+def test_examples_all_precisions(gpu, name):
+    b, g, dt, eps = example(name)
+    ref = oracle.newton(b.masses, b.pos_x, b.pos_y, b.pos_z, g, eps)
+    got = gpu_acc(gpu, b, g, eps, nb.Precision.F64_STRICT)
+    for q in range(3):
+        assert np.array_equal(got[q], ref[q])
+    assert rel_err(gpu_acc(gpu, b, g, eps, nb.Precision.F64), ref).max() <= 1e-12
+    assert rel_err(gpu_acc(gpu, b, g, eps, nb.Precision.F32C), ref).max() <= 1e-5
+
+
+@pytest.mark.parametrize("n", [1000, 4096])
+def test_f64_fast_within_1e12(gpu, n):
+    b = nb.plummer(n, 7 + n)
+    ref = oracle.newton_parallel(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, 0.005)
+    e = rel_err(gpu_acc(gpu, b, 1.0, 0.005, nb.Precision.F64), ref)
+    print(f"n={n} f64 max {e.max():.3e} median {np.median(e):.3e}", flush=True)
+    assert e.max() <= 1e-12
+
+
+def test_config2_size_f64_and_f32c(gpu):
+    """N=65536 Plummer, softening 0.005 (the single-system workload)."""
+    n = 65536
+    b = nb.plummer(n, 2)
+    ref = oracle.newton_parallel(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, 0.005)
+    e64 = rel_err(gpu_acc(gpu, b, 1.0, 0.005, nb.Precision.F64), ref)
+    e32 = rel_err(gpu_acc(gpu, b, 1.0, 0.005, nb.Precision.F32C), ref)
+    print(f"N=65536 f64 max {e64.max():.3e} med {np.median(e64):.3e}; "
+          f"f32c max {e32.max():.3e} med {np.median(e32):.3e}", flush=True)
+    assert e64.max() <= 1e-12
+    assert e32.max() <= 1e-5
+
+
+@pytest.mark.parametrize("integ", INTEG)
+@pytest.mark.parametrize("n", [3, 100, 257])
+def test_strict_trajectories_bit_exact(gpu, integ, n):
+    b = nb.plummer(n, 1000 + n)
+    g, eps, dt, k = 1.0, 0.01, 1e-3, 7
+    ref = oracle.integrate(int(integ), b.masses, b.pos_x, b.pos_y, b.pos_z,
+                           b.vel_x, b.vel_y, b.vel_z, g, eps, dt, k)
+    db = nb.DeviceBodies(gpu, b)
+    grav = nb.GpuNewtonParallelGravity(g, eps * eps, nb.Precision.F64_STRICT)
+    integ.gpu_create(dt).step(gpu, db, grav, 3)   # split calls: carry and
+    integ.gpu_create(dt).step(gpu, db, grav, 4)   # parity across calls
+    out = db.download()
+    for q, f in enumerate(("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z")):
+        assert np.array_equal(getattr(out, f), ref[q]), f
+
+
+@pytest.mark.parametrize("name", sorted(EXAMPLES))
+@pytest.mark.parametrize("integ", INTEG)
+def test_examples_trajectories_strict(gpu, name, integ):
+    b, g, dt, eps = example(name)
+    ref = oracle.integrate(int(integ), b.masses, b.pos_x, b.pos_y, b.pos_z,
+                           b.vel_x, b.vel_y, b.vel_z, g, eps, dt, 50)
+    sim = nb.GpuSimulation(b, nb.SimulationParameters(g, dt, eps),
+                           integrator=integ, precision=nb.Precision.F64_STRICT)
+    sim.step(50)
+    out = sim.snapshot()
+    for q, f in enumerate(("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z")):
+        assert np.array_equal(getattr(out, f), ref[q]), f
+
+
+@pytest.mark.parametrize("integ", INTEG)
+def test_f64_and_f32c_trajectories(gpu, integ):
+    n, g, eps, dt, k = 512, 1.0, 0.01, 1e-3, 20
+    b = nb.plummer(n, 55)
+    ref = oracle.integrate(int(integ), b.masses, b.pos_x, b.pos_y, b.pos_z,
+                           b.vel_x, b.vel_y, b.vel_z, g, eps, dt, k)
+    for prec, tol in ((nb.Precision.F64, 1e-12), (nb.Precision.F32C, 1e-7)):
+        db = nb.DeviceBodies(gpu, b)
+        integ.gpu_create(dt).step(gpu, db, nb.GpuNewtonParallelGravity(g, eps * eps, prec), k)
+        out = db.download()
+        d = np.sqrt((out.pos_x - ref[0]) ** 2 + (out.pos_y - ref[1]) ** 2
+                    + (out.pos_z - ref[2]) ** 2).max()
+        print(f"{integ.name} {prec.name} max position deviation {d:.3e}", flush=True)
+        assert d <= tol
+
+
+def test_golden_config1(gpu):
+    """Config-1 fixture: N=1024 Plummer, eps 0.01, VV dt 1e-3, 100 steps."""
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    f = np.load(os.path.join(here, "config1_plummer1024.npz"))
+    b = nb.Bodies(*[f[k] for k in ("m", "x", "y", "z", "vx", "vy", "vz")])
+    acc = gpu_acc(gpu, b, 1.0, 0.01, nb.Precision.F64_STRICT)
+    for q, k in enumerate(("ax0", "ay0", "az0")):
+        assert np.array_equal(acc[q], f[k])
+    sim = nb.GpuSimulation(b, nb.SimulationParameters(1.0, 1e-3, 0.01),
+                           integrator=nb.IntegratorMethod.VELOCITY_VERLET,
+                           precision=nb.Precision.F64_STRICT)
+    sim.step(100)
+    out = sim.snapshot()
+    for k, fld in (("x100", "pos_x"), ("y100", "pos_y"), ("z100", "pos_z")):
+        assert np.array_equal(getattr(out, fld), f[k])
+
+
+def test_energy_momentum(gpu):
+    b = nb.plummer(2048, 9)
+    sim = nb.GpuSimulation(b, nb.SimulationParameters(1.0, 1e-3, 0.01))
+    em = sim.energy_momentum()
+    K, U = oracle.energy(*b.arrays(), 1.0, 0.01)
+    assert abs(em[0] - K) <= 1e-12 * abs(K)
+    assert abs(em[1] - U) <= 1e-12 * abs(U)
+    assert abs(em[9] - 1.0) <= 1e-12
+    assert np.all(np.abs(em[3:6]) <= 1e-14)
+
+
+def test_unsupported_gravity_methods(gpu):
+    b = nb.plummer(8, 1)
+    db = nb.DeviceBodies(gpu, b)
+    lib = nb.load_library()
+    for gm in (nb.GravityMethod.NEWTON, nb.GravityMethod.BARNES_HUT):
+        rc = lib.nbody_step(gpu.handle, db.handle, 0, int(gm), 1, 1e-3, 1.0, 0.0, 1)
+        assert rc == nb.E_UNSUPPORTED
+    rc = lib.nbody_step(gpu.handle, db.handle, 0, 1, 1, 0.0, 1.0, 0.0, 1)
+    assert rc == nb.E_INVALID
+
+
+def _mixed_systems(count, seed):
+    rng = np.random.default_rng(seed)
+    systems, integ = [], []
+    for s in range(count):
+        n = int(np.exp(rng.uniform(np.log(40), np.log(700))))
+        systems.append(nb.star_system(n, 500 + s) if s % 2 else nb.plummer(n, 500 + s))
+        integ.append(s % 3)
+    return systems, integ
+
+
+@pytest.mark.parametrize("prec", [nb.Precision.F64, nb.Precision.F32C])
+def test_batch_alone_vs_mixed_bitwise(gpu, prec):
+    systems, integ = _mixed_systems(24, 3)
+    dts = [1e-3 if s % 2 else 5e-4 for s in range(len(systems))]
+    mixed = nb.Batch(gpu, systems, integ, dts, 1.0, 1e-4, prec)
+    mixed.step(5)
+    mixed.step(6)
+    mx = mixed.download()
+    for s in (0, 5, 13, 23):
+        # same call pattern: the f32 origin is re-centred at each step call
+        alone = nb.Batch(gpu, [systems[s]], [integ[s]], [dts[s]], 1.0, 1e-4, prec)
+        alone.step(5)
+        alone.step(6)
+        ax = alone.download()
+        o, n = mixed.offsets[s], mixed.n[s]
+        for q in range(6):
+            assert np.array_equal(ax[q], mx[q][o:o + n]), (s, q)
+
+
+def test_batch_matches_oracle(gpu):
+    systems, integ = _mixed_systems(9, 4)
+    bt = nb.Batch(gpu, systems, integ, 1e-3, 1.0, 1e-4, nb.Precision.F64_STRICT)
+    bt.step(4)
+    out = bt.download()
+    for s, b in enumerate(systems):
+        ref = oracle.integrate(integ[s], *b.arrays(), 1.0, 0.01, 1e-3, 4)
+        o, n = bt.offsets[s], bt.n[s]
+        for q in range(6):
+            assert np.array_equal(out[q][o:o + n], ref[q]), (s, q)
