@@ -1,0 +1,162 @@
+"""CPU-only checks of the C ABI library and the host layer (no GPU compute).
+
+* every function include/nbody.h declares is exported by libnbody_amd.so;
+* without a GPU, context/server creation fails with NBODY_E_DEVICE (no CPU
+  fallback exists);
+* argument validation of the host-only entry points;
+* the scenario generators (host code) against their defining properties;
+* the reference's method-selection strings (src/args.rs:125-136, :176-187).
+"""
+import ctypes
+import re
+
+import numpy as np
+import pytest
+
+import nbody_amd as nb
+
+
+def header_functions():
+    text = open(nb.HEADER_PATH).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nbody_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = nb.load_library()
+    names = header_functions()
+    assert len(names) >= 29
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version_string():
+    assert b"gfx950" in nb.load_library().nbody_version()
+
+
+def _no_gpu():
+    try:
+        import torch
+        return not torch.cuda.is_available()
+    except Exception:  # pragma: no cover
+        return True
+
+
+@pytest.mark.skipif(not _no_gpu(), reason="checks the no-GPU error path")
+def test_no_gpu_fails_loudly():
+    with pytest.raises(nb.NbodyError) as e:
+        nb.GpuContext(0)
+    assert e.value.code == nb.E_DEVICE
+    with pytest.raises(nb.NbodyError) as e:
+        nb.Server(capacity_bodies=1024)
+    assert e.value.code == nb.E_DEVICE
+
+
+def test_null_and_invalid_arguments():
+    lib = nb.load_library()
+    h = ctypes.c_void_p()
+    assert lib.nbody_ctx_create(0, None) == nb.E_INVALID
+    assert lib.nbody_ctx_synchronize(None) == nb.E_INVALID
+    assert lib.nbody_step(None, None, 0, 1, 1, 1e-3, 1.0, 0.0, 1) == nb.E_INVALID
+    assert lib.nbody_bodies_create(None, 4, *[None] * 7, ctypes.byref(h)) == nb.E_INVALID
+    assert lib.nbody_ctx_destroy(None) == nb.OK          # null is a no-op
+    assert lib.nbody_bodies_len(None) == 0
+    arrs = [np.empty(4) for _ in range(7)]
+    ptrs = [nb._p(a) for a in arrs]
+    assert lib.nbody_scenario_plummer(0, 1, 1.0, 1.0, 1.0, *ptrs) == nb.E_INVALID
+    assert lib.nbody_scenario_plummer(4, 1, -1.0, 1.0, 1.0, *ptrs) == nb.E_INVALID
+    assert lib.nbody_scenario_colliding(1, 1, 10.0, 0.5, 1.0, *ptrs) == nb.E_INVALID
+    assert lib.nbody_scenario_star_system(4, 1, 1.0, 1e-4, 1e-7, 0.5, 10.0, 5.0, 1.0,
+                                          *ptrs) == nb.E_INVALID   # m_hi < m_lo
+
+
+def test_plummer_properties():
+    b = nb.plummer(20000, 7)
+    m = b.masses
+    assert np.all(m == 1.0 / 20000)
+    pos = np.stack([b.pos_x, b.pos_y, b.pos_z], 1)
+    vel = np.stack([b.vel_x, b.vel_y, b.vel_z], 1)
+    assert np.abs((m[:, None] * pos).sum(0)).max() < 1e-15   # centre-of-mass frame
+    assert np.abs((m[:, None] * vel).sum(0)).max() < 1e-15
+    r = np.linalg.norm(pos, axis=1)
+    # half-mass radius of a Plummer sphere: a / sqrt(2^(2/3) - 1) = 1.3048
+    assert abs(np.median(r) - 1.3048) < 0.03
+    assert r.max() <= 20.0 + 1e-9
+    # virial equilibrium: K = 3*pi/64 for G = M = a = 1 (untruncated)
+    K = 0.5 * (m * (vel ** 2).sum(1)).sum()
+    assert abs(K - 3 * np.pi / 64) < 0.01
+    # speeds below the local escape speed
+    ve = np.sqrt(2.0) * (1 + r * r) ** -0.25
+    assert np.all(np.linalg.norm(vel, axis=1) <= ve * (1 + 1e-12) + 1e-12)
+
+
+def test_plummer_deterministic_and_seeded():
+    a, b, c = nb.plummer(100, 1), nb.plummer(100, 1), nb.plummer(100, 2)
+    assert all(np.array_equal(p, q) for p, q in zip(a.arrays(), b.arrays()))
+    assert not np.array_equal(a.pos_x, c.pos_x)
+
+
+def test_star_system_properties():
+    b = nb.star_system(5000, 3, m_lo=1e-7, m_hi=1e-4, r_lo=0.5, r_hi=10.0,
+                       max_incl_deg=5.0)
+    assert b.masses[0] == 1.0 and b.pos_x[0] == 0.0 and b.vel_x[0] == 0.0
+    m = b.masses[1:]
+    assert m.min() >= 1e-7 and m.max() <= 1e-4
+    # log-uniform: median of log10 mass at the midpoint -5.5
+    assert abs(np.median(np.log10(m)) + 5.5) < 0.05
+    pos = np.stack([b.pos_x, b.pos_y, b.pos_z], 1)[1:]
+    vel = np.stack([b.vel_x, b.vel_y, b.vel_z], 1)[1:]
+    r = np.linalg.norm(pos, axis=1)
+    assert r.min() >= 0.5 and r.max() <= 10.0
+    # circular: speed sqrt(G M / r), velocity perpendicular to the radius
+    assert np.allclose(np.linalg.norm(vel, axis=1), np.sqrt(1.0 / r), rtol=1e-12)
+    assert np.abs((pos * vel).sum(1)).max() < 1e-12
+    # orbit normal within 5 degrees of +z
+    h = np.cross(pos, vel)
+    incl = np.degrees(np.arccos(h[:, 2] / np.linalg.norm(h, axis=1)))
+    assert incl.max() < 5.0 + 1e-9
+
+
+def test_colliding_properties():
+    b = nb.colliding(2000, 5, separation=10.0, rel_speed=0.5)
+    assert np.allclose(b.masses, 1.0 / 2000)
+    left, right = slice(0, 1000), slice(1000, 2000)
+    assert abs(b.pos_x[left].mean() + 5.0) < 1e-12
+    assert abs(b.pos_x[right].mean() - 5.0) < 1e-12
+    assert abs(b.vel_x[left].mean() - 0.25) < 1e-12
+    assert abs(b.vel_x[right].mean() + 0.25) < 1e-12
+
+
+def test_method_strings_match_reference():
+    G, I = nb.GravityMethod, nb.IntegratorMethod
+    assert G.from_str("newton") is G.NEWTON
+    assert G.from_str("Newton-Parallel") is G.NEWTON_PARALLEL   # to_lowercase()
+    assert G.from_str("barnes-hut") is G.BARNES_HUT
+    assert I.from_str("euler") is I.EULER
+    assert I.from_str("velocity-verlet") is I.VELOCITY_VERLET
+    assert I.from_str("RUNGE-KUTTA") is I.RUNGE_KUTTA
+    with pytest.raises(ValueError, match="Invalid gravity method: foo"):
+        G.from_str("foo")
+    with pytest.raises(ValueError, match="Invalid integrator method: rk5"):
+        I.from_str("rk5")
+
+
+def test_gpu_create_selection():
+    p = nb.SimulationParameters(g_constant=2.0, time_step=0.01, softening_factor=0.1)
+    grav = nb.GravityMethod.NEWTON_PARALLEL.gpu_create(p)
+    assert grav.g_constant == 2.0 and grav.eps2 == 0.1 * 0.1   # src/args.rs:115
+    for gm in (nb.GravityMethod.NEWTON, nb.GravityMethod.BARNES_HUT):
+        with pytest.raises(nb.NbodyError) as e:
+            gm.gpu_create(p)
+        assert e.value.code == nb.E_UNSUPPORTED
+    for im, cls in ((nb.IntegratorMethod.EULER, nb.GpuEulerIntegrator),
+                    (nb.IntegratorMethod.VELOCITY_VERLET, nb.GpuVelocityVerletIntegrator),
+                    (nb.IntegratorMethod.RUNGE_KUTTA, nb.GpuRungeKuttaIntegrator)):
+        integ = im.gpu_create(0.01)
+        assert isinstance(integ, cls) and integ.dt == 0.01
+
+
+def test_bodies_length_check():
+    with pytest.raises(ValueError):
+        nb.Bodies(np.ones(3), np.ones(3), np.ones(2), np.ones(3), np.ones(3),
+                  np.ones(3), np.ones(3))
