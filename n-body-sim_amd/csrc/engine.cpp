@@ -20,8 +20,6 @@ void hip_check(hipError_t e, const char* what) {
 
 namespace {
 constexpr int64_t kAlign = 64;  // system ranges start on a wave boundary
-constexpr int kDoubleArrays = 25;
-constexpr int kFloatArrays = 7;
 
 int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 }  // namespace
@@ -57,24 +55,22 @@ Engine::~Engine() {
   hipFree(arena_mem_);
 }
 
-void Engine::alloc_arena() {
-  // Every array gets cap + kPad elements, each array 256-byte aligned.
+// Carves every arena array out of one block starting at `base`; returns the
+// bytes used.  Called once with base = nullptr to size the block, then on the
+// allocation itself, so the size can never disagree with the layout.
+size_t Engine::carve(char* base) {
   const int64_t len = cap_ + kPad;
   const size_t dbytes = round_up(len * 8, 256);
   const size_t fbytes = round_up(len * 4, 256);
-  const size_t total = dbytes * kDoubleArrays + fbytes * kFloatArrays;
-  hip_check(hipMalloc(&arena_mem_, total), "hipMalloc(arena)");
-  // Zero everything once: slack and never-used ranges must hold finite values.
-  hip_check(hipMemsetAsync(arena_mem_, 0, total, stream_), "hipMemset(arena)");
-  char* p = (char*)arena_mem_;
+  size_t used = 0;
   auto D = [&]() {
-    double* r = (double*)p;
-    p += dbytes;
+    double* r = (double*)(base + used);
+    used += dbytes;
     return r;
   };
   auto F = [&]() {
-    float* r = (float*)p;
-    p += fbytes;
+    float* r = (float*)(base + used);
+    used += fbytes;
     return r;
   };
   A_.m = D();
@@ -102,6 +98,34 @@ void Engine::alloc_arena() {
     A_.fz[b] = F();
   }
   A_.cap = cap_;
+  return used;
+}
+
+void Engine::alloc_arena() {
+  // Every array gets cap + kPad elements, each array 256-byte aligned.
+  arena_bytes_ = carve(nullptr);
+  hip_check(hipMalloc(&arena_mem_, arena_bytes_), "hipMalloc(arena)");
+  // Zero everything once: slack and never-used ranges must hold finite values.
+  hip_check(hipMemsetAsync(arena_mem_, 0, arena_bytes_, stream_),
+            "hipMemset(arena)");
+  if (carve((char*)arena_mem_) != arena_bytes_)
+    throw Error(NBODY_E_DEVICE, "arena layout mismatch");
+}
+
+// Host-side guard run on every work list before it is uploaded: each item
+// must name a live system, a tile inside it and a snapshot buffer 0 or 1.
+void Engine::validate_items(const std::vector<WorkItem>& items) const {
+  for (const WorkItem& it : items) {
+    if (it.sys < 0 || it.sys >= max_sys_ || !sys_[it.sys].live)
+      throw Error(NBODY_E_STATE, "work item names no live system");
+    const Sys& s = sys_[it.sys];
+    const int64_t tiles = (s.p.n + kWave - 1) / kWave;
+    if (it.tile < 0 || it.tile >= tiles || (it.src & ~1) != 0 ||
+        it.stage < STAGE_ACC || it.stage > STAGE_POT)
+      throw Error(NBODY_E_STATE, "work item out of range");
+    if (s.off < 0 || s.off + s.span > cap_ || s.span < s.p.n)
+      throw Error(NBODY_E_STATE, "system range outside the arena");
+  }
 }
 
 void Engine::drop_graphs() {
@@ -343,6 +367,7 @@ void Engine::tiles_of(int id, int stage, int src,
 void Engine::run_round(std::vector<WorkItem>& items, bool potential) {
   if (items.empty()) return;
   sort_items(items);
+  validate_items(items);
   ensure_items(items.size());
   hip_check(hipMemcpyAsync(d_items_, items.data(),
                            sizeof(WorkItem) * items.size(),
@@ -386,6 +411,7 @@ void Engine::prepare(const std::vector<int>& ids) {
   }
   upload_table();
   if (items.empty()) return;
+  validate_items(items);
   ensure_items(items.size());
   hip_check(hipMemcpyAsync(d_items_, items.data(),
                            sizeof(WorkItem) * items.size(),
@@ -463,6 +489,7 @@ void Engine::upload_step_lists(const std::vector<int>& ids, int offs[8],
     cnts[r] = (int)l.size();
     all.insert(all.end(), l.begin(), l.end());
   }
+  validate_items(all);
   ensure_items(all.size());
   if (!all.empty()) {
     hip_check(hipMemcpyAsync(d_items_, all.data(), sizeof(WorkItem) * all.size(),
@@ -543,6 +570,7 @@ void Engine::diagnostics(const std::vector<int>& ids, double* out) {
   std::vector<WorkItem> prep;
   for (int id : ids) tiles_of(id, STAGE_ACC, sys_[id].cur ^ 1, prep);
   upload_table();
+  validate_items(prep);
   ensure_items(prep.size());
   hip_check(hipMemcpyAsync(d_items_, prep.data(), sizeof(WorkItem) * prep.size(),
                            hipMemcpyHostToDevice, stream_),

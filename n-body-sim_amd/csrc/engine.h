@@ -99,7 +99,9 @@ class Engine {
     double vcom[3] = {0, 0, 0};  // centre-of-mass velocity
   };
 
+  size_t carve(char* base);
   void alloc_arena();
+  void validate_items(const std::vector<WorkItem>& items) const;
   void upload_table();
   void ensure_items(size_t n);
   void prepare(const std::vector<int>& ids);
@@ -124,6 +126,7 @@ class Engine {
   int max_sys_;
   Arena A_{};
   void* arena_mem_ = nullptr;
+  size_t arena_bytes_ = 0;
   SysDesc* d_sys_ = nullptr;
   std::vector<SysDesc> h_sys_;
   WorkItem* d_items_ = nullptr;

@@ -1,0 +1,144 @@
+"""GPU tests of the served form (continuous batching).
+
+Each result must be the same alone and in the mix: every system is stepped,
+re-centred, snapshotted and retired at the same own-step counts whatever
+else shares its launches, so alone and mixed results are compared bit for bit
+(stronger than the 1e-5 acceleration tolerance of the f32 form).
+"""
+import time
+
+import numpy as np
+import pytest
+
+import nbody_amd as nb
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def drain(srv, ids, timeout_s=120.0):
+    done, snaps = {}, {i: [] for i in ids}
+    t_end = time.time() + timeout_s
+    while len(done) < len(ids):
+        assert time.time() < t_end, "server did not finish in time"
+        r = srv.poll(timeout_ms=1000, capacity=1 << 16)
+        if r is None:
+            continue
+        ev, data = r
+        if ev["kind"] == nb.EVENT_SNAPSHOT:
+            snaps[ev["id"]].append((ev["step"], data))
+        else:
+            done[ev["id"]] = (ev, data)
+    return done, snaps
+
+
+def requests(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(32), np.log(900))))
+        b = nb.plummer(n, 7000 + k) if k % 2 == 0 else nb.star_system(n, 7000 + k)
+        out.append(dict(bodies=b, integrator=nb.IntegratorMethod(k % 3),
+                        dt=(1e-3, 5e-4)[k % 2], steps=int(rng.integers(10, 60)),
+                        eps2=1e-4))
+    return out
+
+
+def run_server(reqs, precision, snapshots=True, capacity=1 << 16):
+    srv = nb.Server(capacity_bodies=capacity, max_systems=256, snapshot_every=10,
+                    precision=precision)
+    ids = [srv.submit(r["bodies"], r["integrator"], r["dt"], r["steps"],
+                      eps2=r["eps2"], want_snapshots=snapshots) for r in reqs]
+    done, snaps = drain(srv, ids)
+    stats = srv.stats()
+    srv.close()
+    return ids, done, snaps, stats
+
+
+@pytest.mark.parametrize("precision", [nb.Precision.F64_STRICT, nb.Precision.F32C])
+def test_alone_equals_mixed(gpu, precision):
+    reqs = requests(20, 1)
+    ids, done, snaps, stats = run_server(reqs, precision)
+    assert stats["completed"] == len(reqs) and stats["failed"] == 0
+    for k in (0, 7, 13, 19):
+        aid, adone, asnaps, _ = run_server([reqs[k]], precision)
+        ev_m, data_m = done[ids[k]]
+        ev_a, data_a = adone[aid[0]]
+        assert ev_m["kind"] == ev_a["kind"] == nb.EVENT_DONE
+        assert ev_m["step"] == ev_a["step"] == reqs[k]["steps"]
+        for q in range(6):
+            assert np.array_equal(data_m[q], data_a[q]), (k, q)
+        assert [s for s, _ in snaps[ids[k]]] == [s for s, _ in asnaps[aid[0]]]
+        for (_, dm), (_, da) in zip(snaps[ids[k]], asnaps[aid[0]]):
+            for q in range(3):
+                assert np.array_equal(dm[q], da[q])
+
+
+def test_strict_server_matches_oracle(gpu):
+    reqs = requests(6, 2)
+    ids, done, snaps, _ = run_server(reqs, nb.Precision.F64_STRICT)
+    for rid, r in zip(ids, reqs):
+        b = r["bodies"]
+        ref = oracle.integrate(int(r["integrator"]), *b.arrays(), 1.0, 0.01, r["dt"],
+                               r["steps"])
+        ev, data = done[rid]
+        for q in range(6):
+            assert np.array_equal(data[q], ref[q])
+        # snapshots every 10 own steps, in order, strictly before the end
+        steps = [s for s, _ in snaps[rid]]
+        assert steps == list(range(10, r["steps"], 10))[:len(steps)]
+        assert steps == sorted(steps)
+
+
+def test_admission_by_memory_and_backlog(gpu):
+    # arena of 1024 bodies: requests of 600 bodies cannot be resident together
+    reqs = [dict(bodies=nb.plummer(600, 30 + k), integrator=nb.IntegratorMethod.EULER,
+                 dt=1e-3, steps=20, eps2=1e-4) for k in range(3)]
+    ids, done, _, stats = run_server(reqs, nb.Precision.F32C, snapshots=False,
+                                     capacity=1024)
+    assert stats["completed"] == 3
+    evs = [done[i][0] for i in ids]
+    # served one after another: each admitted no earlier than the previous ended
+    for a, b in zip(evs, evs[1:]):
+        assert b["t_admit"] >= a["t_event"] - 1e-3
+
+
+def test_reject_oversized_and_bad_requests(gpu):
+    srv = nb.Server(capacity_bodies=256, max_systems=4)
+    with pytest.raises(nb.NbodyError) as e:
+        srv.submit(nb.plummer(1000, 1), nb.IntegratorMethod.EULER, 1e-3, 10)
+    assert e.value.code == nb.E_NOSPACE
+    b = nb.plummer(16, 1)
+    with pytest.raises(nb.NbodyError) as e:
+        srv.submit(b, nb.IntegratorMethod.EULER, 0.0, 10)
+    assert e.value.code == nb.E_INVALID
+    bad = b.copy()
+    bad.pos_x[3] = np.nan
+    with pytest.raises(nb.NbodyError) as e:
+        srv.submit(bad, nb.IntegratorMethod.EULER, 1e-3, 10)
+    assert e.value.code == nb.E_INVALID
+    with pytest.raises(nb.NbodyError) as e:
+        srv.submit(b, nb.IntegratorMethod.EULER, 1e-3, 10,
+                   gravity=nb.GravityMethod.BARNES_HUT)
+    assert e.value.code == nb.E_UNSUPPORTED
+    srv.close()
+
+
+def test_cancel(gpu):
+    srv = nb.Server(capacity_bodies=4096, max_systems=8, snapshot_every=10)
+    big = srv.submit(nb.plummer(2000, 3), nb.IntegratorMethod.RUNGE_KUTTA, 1e-3, 100000)
+    queued = srv.submit(nb.plummer(3000, 4), nb.IntegratorMethod.EULER, 1e-3, 10)
+    srv.cancel(queued)     # cannot fit beside `big`: still queued
+    time.sleep(0.2)
+    srv.cancel(big)        # resident: retired at the next epoch boundary
+    kinds = {}
+    t_end = time.time() + 60
+    while len(kinds) < 2 and time.time() < t_end:
+        r = srv.poll(timeout_ms=500)
+        if r and r[0]["kind"] in (nb.EVENT_CANCELLED, nb.EVENT_DONE):
+            kinds[r[0]["id"]] = r[0]["kind"]
+    assert kinds == {big: nb.EVENT_CANCELLED, queued: nb.EVENT_CANCELLED}
+    with pytest.raises(nb.NbodyError) as e:
+        srv.cancel(12345)
+    assert e.value.code == nb.E_NOTFOUND
+    srv.close()

@@ -1,0 +1,69 @@
+"""CPU model of the force kernel's memory-access pattern (no GPU needed).
+
+Replays, for every (n, tile, chunk size), the source regions the kernel walks
+(kernels.hip: make_bounds and the sweep loops, including the prefetch of one
+chunk ahead) and checks that every source index stays inside the system's
+arena span plus the arena's zeroed slack, that the regions cover the
+system's sources exactly once in index order, and that the masked regions
+are the only ones that can contain j == i or j >= n.
+"""
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TYPES = open(os.path.join(ROOT, "n-body-sim_amd", "csrc", "nbody_types.h")).read()
+K_PAD = int(re.search(r"kPad = (\d+);", TYPES).group(1))
+K_WAVE = int(re.search(r"kWave = (\d+);", TYPES).group(1))
+ALIGN = 64  # engine.cpp kAlign: systems start on 64-body boundaries
+
+
+def bounds(off, n, t0, u):
+    e_up = off + (n + u - 1) // u * u
+    nfull = off + n // u * u
+    b1 = min(t0 + K_WAVE, e_up)
+    c1 = max(b1, nfull)
+    return [(off, t0, False), (t0, b1, True), (b1, c1, False), (c1, e_up, True)]
+
+
+def accessed(j0, j1, u, prefetch):
+    if j0 >= j1:
+        return []
+    idx = list(range(j0, j1))
+    if prefetch:  # sweep_f64 / sweep_f32 read the chunk after each chunk
+        idx += list(range(j1, j1 + u))
+    return idx
+
+
+@pytest.mark.parametrize("u,prefetch", [(4, False), (4, True), (8, True)])
+def test_source_indices_in_bounds_and_ordered(u, prefetch):
+    for n in list(range(1, 140)) + [255, 256, 257, 1000, 1023, 1024, 1025, 4099]:
+        span = (n + ALIGN - 1) // ALIGN * ALIGN
+        off = 3 * ALIGN                     # system placed last in the arena
+        cap = off + span
+        tiles = (n + K_WAVE - 1) // K_WAVE
+        for tile in range(tiles):
+            t0 = off + tile * K_WAVE
+            regions = bounds(off, n, t0, u)
+            seen = []
+            for j0, j1, masked in regions:
+                assert (j0 - off) % u == 0 and (j1 - off) % u == 0
+                for j in accessed(j0, j1, u, prefetch):
+                    assert 0 <= j < cap + K_PAD, (n, tile, j)
+                for j in range(j0, j1):
+                    seen.append(j)
+                    needs_mask = j >= off + n or t0 <= j < t0 + K_WAVE
+                    if needs_mask:
+                        assert masked, (n, tile, j)
+            # every source visited once, in index order (reference j order)
+            assert seen == list(range(off, off + (n + u - 1) // u * u))
+            # the lanes' own loads: il = min(i, end - 1)
+            for lane in range(K_WAVE):
+                il = min(t0 + lane, off + n - 1)
+                assert off <= il < off + n
+
+
+def test_slack_covers_largest_prefetch():
+    assert K_PAD >= 8          # one f32 chunk (U = 8) past a rounded-up end
+    assert ALIGN % 8 == 0 and K_WAVE % 8 == 0
