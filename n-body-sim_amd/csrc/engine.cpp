@@ -63,9 +63,9 @@ size_t Engine::carve(char* base) {
   const size_t dbytes = round_up(len * 8, 256);
   const size_t fbytes = round_up(len * 4, 256);
   size_t used = 0;
-  auto D = [&]() {
+  auto D = [&](int count = 1) {
     double* r = (double*)(base + used);
-    used += dbytes;
+    used += dbytes * count;
     return r;
   };
   auto F = [&]() {
@@ -91,6 +91,10 @@ size_t Engine::carve(char* base) {
   A_.az = D();
   for (int q = 0; q < 9; ++q) A_.rk[q] = D();
   A_.phi = D();
+  A_.px = D(kMaxSplit);
+  A_.py = D(kMaxSplit);
+  A_.pz = D(kMaxSplit);
+  A_.pstride = (int64_t)(dbytes / sizeof(double));
   A_.gm32 = F();
   for (int b = 0; b < 2; ++b) {
     A_.fx[b] = F();
@@ -121,7 +125,8 @@ void Engine::validate_items(const std::vector<WorkItem>& items) const {
     const Sys& s = sys_[it.sys];
     const int64_t tiles = (s.p.n + kWave - 1) / kWave;
     if (it.tile < 0 || it.tile >= tiles || (it.src & ~1) != 0 ||
-        it.stage < STAGE_ACC || it.stage > STAGE_POT)
+        it.stage < STAGE_ACC || it.stage > STAGE_POT || it.part < 0 ||
+        it.part >= std::max(1, split_of(it.sys)))
       throw Error(NBODY_E_STATE, "work item out of range");
     if (s.off < 0 || s.off + s.span > cap_ || s.span < s.p.n)
       throw Error(NBODY_E_STATE, "system range outside the arena");
@@ -306,6 +311,7 @@ void Engine::set_mode(int mode) {
     s.vv_primed = false;
     s.prepared = false;
   }
+  table_dirty_ = true;  // split counts depend on the mode
   drop_graphs();
 }
 
@@ -326,7 +332,23 @@ SysDesc Engine::desc_of(int id) const {
   d.eps2 = s.p.eps2;
   d.g = s.p.g;
   d.eps2f = (float)s.p.eps2;
+  d.nsplit = split_of(id);
   return d;
+}
+
+// Source-range parts per tile for one system.  Depends only on the system's
+// own size and the arithmetic mode (never on what shares the launch), so a
+// system's summation order -- and its bits -- are the same alone or in a
+// batch.  Aim for about kTargetWaves waves per system so a single system
+// fills the GPU (4 waves per SIMD on 256 CUs), with at least kMinPart sources
+// per part; strict mode keeps the reference's single sequential sum.
+int Engine::split_of(int id) const {
+  if (mode_ == MODE_F64_STRICT) return 1;
+  constexpr int64_t kTargetWaves = 4096, kMinPart = 512;
+  const int64_t n = sys_[id].p.n;
+  const int64_t tiles = (n + kWave - 1) / kWave;
+  int64_t k = std::min(kTargetWaves / tiles, n / kMinPart);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(k, kMaxSplit));
 }
 
 void Engine::upload_table() {
@@ -357,28 +379,44 @@ void Engine::ensure_items(size_t n) {
             "hipMalloc(work list)");
 }
 
-void Engine::tiles_of(int id, int stage, int src,
-                      std::vector<WorkItem>& out) const {
+// Force items of one system: one per (tile, part).  With combine != null the
+// system's combine items (one per tile) are appended there when it is split.
+void Engine::tiles_of(int id, int stage, int src, std::vector<WorkItem>& out,
+                      std::vector<WorkItem>* combine) const {
   const int64_t tiles = (sys_[id].p.n + kWave - 1) / kWave;
+  const int parts = combine ? split_of(id) : 1;
   for (int64_t t = 0; t < tiles; ++t)
-    out.push_back(WorkItem{id, (int32_t)t, stage, src});
+    for (int q = 0; q < parts; ++q)
+      out.push_back(WorkItem{id, (int32_t)t, (int16_t)stage, (int8_t)src,
+                             (int8_t)q});
+  if (combine && parts > 1)
+    for (int64_t t = 0; t < tiles; ++t)
+      combine->push_back(
+          WorkItem{id, (int32_t)t, (int16_t)stage, (int8_t)src, (int8_t)0});
 }
 
-void Engine::run_round(std::vector<WorkItem>& items, bool potential) {
+// One force round outside the stepping loop (prime, plain accelerations,
+// potential): uploads the force items and, for split systems, the combine
+// items, launches both, and waits.
+void Engine::run_round(std::vector<WorkItem>& items,
+                       std::vector<WorkItem>& combine, bool potential) {
   if (items.empty()) return;
   sort_items(items);
   validate_items(items);
-  ensure_items(items.size());
-  hip_check(hipMemcpyAsync(d_items_, items.data(),
-                           sizeof(WorkItem) * items.size(),
+  validate_items(combine);
+  std::vector<WorkItem> all(items);
+  all.insert(all.end(), combine.begin(), combine.end());
+  ensure_items(all.size());
+  hip_check(hipMemcpyAsync(d_items_, all.data(), sizeof(WorkItem) * all.size(),
                            hipMemcpyHostToDevice, stream_),
             "hipMemcpy(work list)");
+  const int nf = (int)items.size(), nc = (int)combine.size();
   hipError_t e = potential
-                     ? launch_potential(A_, d_items_, (int)items.size(),
-                                        d_sys_, stream_)
-                     : launch_force(mode_, A_, d_items_, (int)items.size(),
-                                    d_sys_, stream_);
+                     ? launch_potential(A_, d_items_, nf, d_sys_, stream_)
+                     : launch_force(mode_, A_, d_items_, nf, d_sys_, stream_);
   hip_check(e, "force launch");
+  hip_check(launch_combine(mode_, A_, d_items_ + nf, nc, d_sys_, stream_),
+            "combine launch");
   // the pageable host list must not change before the copy is consumed
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
@@ -427,24 +465,27 @@ void Engine::set_rebase_interval(int id, int64_t every) {
   sys_[id].rebase_every = every;
 }
 
-// Work lists of one step for the listed systems: lists[r] for rounds 0..3,
-// with each system's snapshot parity as it will be when that round runs,
-// given `flip` extra steps already taken by Euler / velocity-Verlet systems
-// (RK4 systems return to the same buffer after their four stages).
+// Work lists of one step for the listed systems: force[r] and combine[r]
+// for rounds 0..3, with each system's snapshot parity as it will be when that
+// round runs, given `flip` extra steps already taken by Euler /
+// velocity-Verlet systems (RK4 systems return to the same buffer after their
+// four stages).
 void Engine::step_lists(const std::vector<int>& ids, int flip,
-                        std::vector<WorkItem> (&lists)[4]) const {
+                        std::vector<WorkItem> (&force)[4],
+                        std::vector<WorkItem> (&combine)[4]) const {
   for (int r = 0; r < 4; ++r) {
-    lists[r].clear();
+    force[r].clear();
+    combine[r].clear();
     for (int id : ids) {
       const Sys& s = sys_[id];
       if (s.p.integ == INTEG_RK4) {
-        tiles_of(id, STAGE_RK0 + r, s.cur ^ (r & 1), lists[r]);
+        tiles_of(id, STAGE_RK0 + r, s.cur ^ (r & 1), force[r], &combine[r]);
       } else if (r == 0) {
         tiles_of(id, s.p.integ == INTEG_EULER ? STAGE_EULER : STAGE_VV,
-                 s.cur ^ flip, lists[r]);
+                 s.cur ^ flip, force[r], &combine[r]);
       }
     }
-    sort_items(lists[r]);
+    sort_items(force[r]);
   }
 }
 
@@ -458,14 +499,14 @@ void Engine::sort_items(std::vector<WorkItem>& items) const {
 
 // Primes velocity-Verlet systems that carry no acceleration yet.
 void Engine::prime(const std::vector<int>& ids) {
-  std::vector<WorkItem> items;
+  std::vector<WorkItem> items, combine;
   for (int id : ids) {
     const Sys& s = sys_[id];
     if (s.p.integ == INTEG_VV && !s.vv_primed)
-      tiles_of(id, STAGE_VV_PRIME, s.cur, items);
+      tiles_of(id, STAGE_VV_PRIME, s.cur, items, &combine);
   }
   if (items.empty()) return;
-  run_round(items);
+  run_round(items, combine);
   for (int id : ids) {
     Sys& s = sys_[id];
     if (s.p.integ == INTEG_VV && !s.vv_primed) {
@@ -475,18 +516,19 @@ void Engine::prime(const std::vector<int>& ids) {
   }
 }
 
-// Uploads the step-A and step-B lists (A: current parity, B: after one
-// step) into one device buffer; returns the device offsets and counts.
-void Engine::upload_step_lists(const std::vector<int>& ids, int offs[8],
-                               int cnts[8]) {
-  std::vector<WorkItem> la[4], lb[4];
-  step_lists(ids, 0, la);
-  step_lists(ids, 1, lb);
+// Uploads the lists of step A (current parity) and step B (after one step)
+// into one device buffer.  Slot k = which*8 + round*2 + (0 force, 1 combine).
+void Engine::upload_step_lists(const std::vector<int>& ids, StepLists& L) {
+  std::vector<WorkItem> fa[4], ca[4], fb[4], cb[4];
+  step_lists(ids, 0, fa, ca);
+  step_lists(ids, 1, fb, cb);
   std::vector<WorkItem> all;
-  for (int r = 0; r < 8; ++r) {
-    const std::vector<WorkItem>& l = r < 4 ? la[r] : lb[r - 4];
-    offs[r] = (int)all.size();
-    cnts[r] = (int)l.size();
+  for (int k = 0; k < 16; ++k) {
+    const int which = k / 8, r = (k % 8) / 2, kind = k % 2;
+    const std::vector<WorkItem>& l =
+        which == 0 ? (kind ? ca[r] : fa[r]) : (kind ? cb[r] : fb[r]);
+    L.off[k] = (int)all.size();
+    L.cnt[k] = (int)l.size();
     all.insert(all.end(), l.begin(), l.end());
   }
   validate_items(all);
@@ -499,13 +541,16 @@ void Engine::upload_step_lists(const std::vector<int>& ids, int offs[8],
   }
 }
 
-void Engine::launch_step(const int offs[8], const int cnts[8], int which) {
+void Engine::launch_step(const StepLists& L, int which) {
   for (int r = 0; r < 4; ++r) {
-    const int k = which * 4 + r;
-    if (cnts[k] == 0) continue;
-    hip_check(launch_force(mode_, A_, d_items_ + offs[k], cnts[k], d_sys_,
+    const int k = which * 8 + r * 2;
+    if (L.cnt[k] == 0) continue;
+    hip_check(launch_force(mode_, A_, d_items_ + L.off[k], L.cnt[k], d_sys_,
                            stream_),
               "force launch");
+    hip_check(launch_combine(mode_, A_, d_items_ + L.off[k + 1], L.cnt[k + 1],
+                             d_sys_, stream_),
+              "combine launch");
   }
 }
 
@@ -523,9 +568,9 @@ void Engine::step(const std::vector<int>& ids, int64_t nsteps) {
   if (nsteps == 0 || ids.empty()) return;
   prepare(ids);
   prime(ids);
-  int offs[8], cnts[8];
-  upload_step_lists(ids, offs, cnts);
-  for (int64_t t = 0; t < nsteps; ++t) launch_step(offs, cnts, (int)(t & 1));
+  StepLists L;
+  upload_step_lists(ids, L);
+  for (int64_t t = 0; t < nsteps; ++t) launch_step(L, (int)(t & 1));
   finish_steps(ids, nsteps);
 }
 
@@ -537,9 +582,9 @@ void Engine::accelerations(int id, double* ax, double* ay, double* az) {
     s.prepared = false;
   }
   prepare({id});
-  std::vector<WorkItem> items;
-  tiles_of(id, STAGE_ACC, s.cur, items);
-  run_round(items);
+  std::vector<WorkItem> items, combine;
+  tiles_of(id, STAGE_ACC, s.cur, items, &combine);
+  run_round(items, combine);
   s.vv_primed = false;  // the carried acceleration was overwritten
   if (ax || ay || az) {
     const size_t bytes = sizeof(double) * s.p.n;
@@ -578,8 +623,9 @@ void Engine::diagnostics(const std::vector<int>& ids, double* out) {
   hip_check(launch_prepare(A_, d_items_, (int)prep.size(), d_sys_, stream_),
             "prepare launch");
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  for (int id : ids) tiles_of(id, STAGE_POT, sys_[id].cur ^ 1, items);
-  run_round(items, /*potential=*/true);
+  std::vector<WorkItem> combine;
+  for (int id : ids) tiles_of(id, STAGE_POT, sys_[id].cur ^ 1, items, &combine);
+  run_round(items, combine, /*potential=*/true);
   (void)saved;
   hip_check(hipMemcpyAsync(d_list_, list.data(), sizeof(int32_t) * list.size(),
                            hipMemcpyHostToDevice, stream_),
@@ -636,18 +682,18 @@ void Engine::step_graph(const std::vector<int>& ids, int64_t nsteps) {
   if (nsteps == 0 || ids.empty()) return;
   prepare(ids);
   prime(ids);
-  int offs[8], cnts[8];
-  upload_step_lists(ids, offs, cnts);
+  StepLists L;
+  upload_step_lists(ids, L);
   int64_t key = (int64_t)(uintptr_t)d_items_ * 31 + mode_;
-  for (int k = 0; k < 8; ++k) key = key * 1000003 + cnts[k];
+  for (int k = 0; k < 16; ++k) key = key * 1000003 + L.cnt[k];
   if (key != gkey_ || ids != gids_ || !gexec_[0]) {
     drop_graphs();
     hipGraph_t g = nullptr;
     hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
               "hipStreamBeginCapture");
     try {
-      launch_step(offs, cnts, 0);
-      launch_step(offs, cnts, 1);
+      launch_step(L, 0);
+      launch_step(L, 1);
     } catch (...) {
       hipStreamEndCapture(stream_, &g);
       if (g) hipGraphDestroy(g);
@@ -662,7 +708,7 @@ void Engine::step_graph(const std::vector<int>& ids, int64_t nsteps) {
   }
   for (int64_t t = 0; t + 1 < nsteps; t += 2)
     hip_check(hipGraphLaunch(gexec_[0], stream_), "hipGraphLaunch");
-  if (nsteps & 1) launch_step(offs, cnts, 0);
+  if (nsteps & 1) launch_step(L, 0);
   finish_steps(ids, nsteps);
 }
 

@@ -107,15 +107,22 @@ class Engine {
   void prepare(const std::vector<int>& ids);
   void set_origin(Sys& s, SysDesc& d) const;
   SysDesc desc_of(int id) const;
-  void tiles_of(int id, int stage, int src, std::vector<WorkItem>& out) const;
-  void run_round(std::vector<WorkItem>& items, bool potential = false);
+  void tiles_of(int id, int stage, int src, std::vector<WorkItem>& out,
+                std::vector<WorkItem>* combine = nullptr) const;
+  int split_of(int id) const;
+  struct StepLists {  // device offsets/counts: which*8 + round*2 + kind
+    int off[16] = {};
+    int cnt[16] = {};
+  };
+  void run_round(std::vector<WorkItem>& items, std::vector<WorkItem>& combine,
+                 bool potential = false);
   void sort_items(std::vector<WorkItem>& items) const;
   void step_lists(const std::vector<int>& ids, int flip,
-                  std::vector<WorkItem> (&lists)[4]) const;
+                  std::vector<WorkItem> (&force)[4],
+                  std::vector<WorkItem> (&combine)[4]) const;
   void prime(const std::vector<int>& ids);
-  void upload_step_lists(const std::vector<int>& ids, int offs[8],
-                         int cnts[8]);
-  void launch_step(const int offs[8], const int cnts[8], int which);
+  void upload_step_lists(const std::vector<int>& ids, StepLists& L);
+  void launch_step(const StepLists& L, int which);
   void finish_steps(const std::vector<int>& ids, int64_t nsteps);
   void check_id(int id) const;
 

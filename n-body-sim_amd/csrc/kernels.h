@@ -10,6 +10,10 @@ namespace nbody {
 // fused into the epilogue.  mode: Mode.  items/sys: device pointers.
 hipError_t launch_force(int mode, const Arena& A, const WorkItem* items,
                         int nitems, const SysDesc* sys, hipStream_t s);
+// Sums the partial results of split source ranges in part order, then runs
+// each target's stage epilogue (or stores its potential for STAGE_POT).
+hipError_t launch_combine(int mode, const Arena& A, const WorkItem* items,
+                          int nitems, const SysDesc* sys, hipStream_t s);
 // Per-body softened potential sum_j g*m_j / sqrt(r^2 + eps2) into A.phi (f64).
 hipError_t launch_potential(const Arena& A, const WorkItem* items, int nitems,
                             const SysDesc* sys, hipStream_t s);

@@ -82,6 +82,29 @@ __device__ __forceinline__ Bounds make_bounds(int64_t off, int64_t n,
   return b;
 }
 
+// Restricts the regions to the source range of one part: the system's
+// chunks [0, nchunks) are cut into nsplit contiguous runs, part p taking
+// chunks [p*nchunks/nsplit, (p+1)*nchunks/nsplit).  Parts cover every chunk
+// once, in order.
+template <int U>
+__device__ __forceinline__ Bounds clip_part(Bounds b, int64_t off, int64_t n,
+                                            int nsplit, int part) {
+  if (nsplit <= 1) return b;
+  const int64_t nchunks = (n + U - 1) / U;
+  const int64_t lo = off + (nchunks * part / nsplit) * U;
+  const int64_t hi = off + (nchunks * (part + 1) / nsplit) * U;
+  auto clip = [&](int64_t& a, int64_t& e) {
+    a = max(a, lo);
+    e = min(e, hi);
+    if (e < a) e = a;
+  };
+  clip(b.a0, b.a1);
+  clip(b.b0, b.b1);
+  clip(b.c0, b.c1);
+  clip(b.d0, b.d1);
+  return b;
+}
+
 // ---- binary64, reference operation order (no contraction, IEEE sqrt/div).
 template <bool MASK>
 __device__ __forceinline__ void sweep_strict(const cf64* X, const cf64* Y,
@@ -444,7 +467,8 @@ __global__ __launch_bounds__(kBlock) void k_force(Arena A,
     const float xi = A.fx[src][il], yi = A.fy[src][il], zi = A.fz[src][il];
     const float e2 = sd.eps2f;
     Kahan3 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const Bounds b = make_bounds<8>(off, n, t0);
+    const Bounds b =
+        clip_part<8>(make_bounds<8>(off, n, t0), off, n, sd.nsplit, it.part);
     sweep_f32<false>(X, Y, Z, G, b.a0, b.a1, i, end, xi, yi, zi, e2, acc);
     sweep_f32<true>(X, Y, Z, G, b.b0, b.b1, i, end, xi, yi, zi, e2, acc);
     sweep_f32<false>(X, Y, Z, G, b.c0, b.c1, i, end, xi, yi, zi, e2, acc);
@@ -459,7 +483,8 @@ __global__ __launch_bounds__(kBlock) void k_force(Arena A,
     const cf64* G = (const cf64*)A.gm64;
     const double xi = A.sx[src][il], yi = A.sy[src][il], zi = A.sz[src][il];
     const double e2 = sd.eps2;
-    const Bounds b = make_bounds<4>(off, n, t0);
+    const Bounds b =
+        clip_part<4>(make_bounds<4>(off, n, t0), off, n, sd.nsplit, it.part);
     if (MODE == MODE_F64_STRICT) {
       sweep_strict<false>(X, Y, Z, G, b.a0, b.a1, i, end, xi, yi, zi, e2, ax,
                           ay, az);
@@ -481,11 +506,52 @@ __global__ __launch_bounds__(kBlock) void k_force(Arena A,
     }
   }
   if (!active) return;
+  if (sd.nsplit > 1) {  // partial sum; k_combine finishes the target
+    const int64_t q = (int64_t)it.part * A.pstride + i;
+    A.px[q] = ax;
+    if (!POT) {
+      A.py[q] = ay;
+      A.pz[q] = az;
+    }
+    return;
+  }
   if (POT) {
     A.phi[i] = ax;
     return;
   }
   epilogue<MODE>(A, sd, it.stage, src ^ 1, i, ax, ay, az);
+}
+
+// Adds the parts' partial sums of each target in part order (a fixed order,
+// so results are reproducible) and runs the target's stage epilogue.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_combine(Arena A,
+                                                    const WorkItem* items,
+                                                    int nitems,
+                                                    const SysDesc* sysd) {
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int idx = (int)blockIdx.x * kWavesPerBlock + wave;
+  if (idx >= nitems) return;
+  const WorkItem it = load_uniform(items + idx);
+  const SysDesc sd = load_uniform(sysd + it.sys);
+  const int64_t end = sd.off + sd.n;
+  const int64_t i = sd.off + (int64_t)it.tile * kWave +
+                    (int64_t)(threadIdx.x & (kWave - 1));
+  if (i >= end) return;
+  const int64_t st = A.pstride;
+  double ax = A.px[i];
+  if (it.stage == STAGE_POT) {
+    for (int p = 1; p < sd.nsplit; ++p) ax = ax + A.px[p * st + i];
+    A.phi[i] = ax;
+    return;
+  }
+  double ay = A.py[i], az = A.pz[i];
+  for (int p = 1; p < sd.nsplit; ++p) {
+    ax = ax + A.px[p * st + i];
+    ay = ay + A.py[p * st + i];
+    az = az + A.pz[p * st + i];
+  }
+  epilogue<MODE>(A, sd, it.stage, it.src ^ 1, i, ax, ay, az);
 }
 
 // Builds source snapshot it.src of each body from the master state:
@@ -604,6 +670,29 @@ hipError_t launch_force(int mode, const Arena& A, const WorkItem* items,
     case MODE_F32C:
       hipLaunchKernelGGL((k_force<MODE_F32C, false>), grid, block, 0, s, A,
                          items, nitems, sys);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_combine(int mode, const Arena& A, const WorkItem* items,
+                          int nitems, const SysDesc* sys, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  const dim3 grid(blocks_for(nitems)), block(kBlock);
+  switch (mode) {
+    case MODE_F64_STRICT:
+      hipLaunchKernelGGL(k_combine<MODE_F64_STRICT>, grid, block, 0, s, A,
+                         items, nitems, sys);
+      break;
+    case MODE_F64:
+      hipLaunchKernelGGL(k_combine<MODE_F64>, grid, block, 0, s, A, items,
+                         nitems, sys);
+      break;
+    case MODE_F32C:
+      hipLaunchKernelGGL(k_combine<MODE_F32C>, grid, block, 0, s, A, items,
+                         nitems, sys);
       break;
     default:
       return hipErrorInvalidValue;

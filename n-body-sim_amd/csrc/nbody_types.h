@@ -9,6 +9,7 @@
 //                                 output of a plain force evaluation
 //   rk[9]                         RK4 running sums and next-stage velocity
 //   phi                           per-body potential (energy diagnostics)
+//   px/py/pz[kMaxSplit]           partial sums of split source ranges
 // Systems occupy disjoint [off, off+n) ranges of every array.  Every array has
 // kPad elements of zeroed slack past the arena end so a source chunk that
 // straddles the last body stays in bounds.
@@ -20,6 +21,7 @@ namespace nbody {
 constexpr int kWave = 64;        // gfx950 wavefront
 constexpr int kBlock = 256;      // 4 waves per workgroup, one work item each
 constexpr int kPad = 128;        // slack elements after the arena
+constexpr int kMaxSplit = 8;     // source-range parts per target tile
 
 enum Mode : int32_t {            // force arithmetic (public NBODY_PRECISION_*)
   MODE_F64_STRICT = 0,           // reference operation order, no contraction,
@@ -53,15 +55,18 @@ struct SysDesc {       // one per resident system (80 B)
   double g;            // gravitational constant
   double ox, oy, oz;   // origin of the f32 snapshot (f64)
   float eps2f;         // eps2 rounded to f32
-  int32_t pad_[3];
+  int32_t nsplit;      // source-range parts per tile (1: no split)
+  int32_t pad_[2];
 };
 static_assert(sizeof(SysDesc) == 80, "SysDesc layout");
 
 struct WorkItem {      // one wave: 64 consecutive targets of one system
   int32_t sys;         // index into the SysDesc table
   int32_t tile;        // targets [off + 64*tile, off + 64*tile + 64) clipped to n
-  int32_t stage;       // Stage
-  int32_t src;         // snapshot buffer read (0/1); the epilogue writes src^1
+  int16_t stage;       // Stage
+  int8_t src;          // snapshot buffer read (0/1); the epilogue writes src^1
+  int8_t part;         // source-range part (0 .. nsplit-1)
+  int32_t pad_ = 0;
 };
 static_assert(sizeof(WorkItem) == 16, "WorkItem layout");
 
@@ -74,6 +79,8 @@ struct Arena {         // device pointers, passed by value to every kernel
   double *ax, *ay, *az;
   double *rk[9];
   double *phi;
+  double *px, *py, *pz; // per-part partial sums: [part * pstride + i]
+  int64_t pstride;     // cap + kPad
   int64_t cap;         // bodies (excluding the kPad slack)
 };
 

@@ -44,6 +44,29 @@ def rel_err(a, ref):
     return np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
 
 
+def pair_scale(b, g, eps):
+    """S_i = sum_j |a_ij|: the scale of the terms summed into a_i (numpy)."""
+    p = np.stack([b.pos_x, b.pos_y, b.pos_z], 1)
+    d = p[None] - p[:, None]
+    r2 = (d * d).sum(-1) + eps * eps
+    np.fill_diagonal(r2, 1.0)
+    w = g * b.masses[None] * np.sqrt((d * d).sum(-1)) / r2 ** 1.5
+    np.fill_diagonal(w, 0.0)
+    return w.sum(1)
+
+
+def check_small(got, ref, b, g, eps, tol):
+    """Relative error where the net force is well conditioned
+    (|a_i| >= 1e-3 S_i); error normalised by S_i where the terms cancel."""
+    a, r = np.stack(got, 1), np.stack(ref, 1)
+    d = np.linalg.norm(a - r, axis=1)
+    nr = np.linalg.norm(r, axis=1)
+    S = pair_scale(b, g, eps)
+    well = nr >= 1e-3 * S
+    assert np.all(d[well] <= tol * nr[well])
+    assert np.all(d[~well] <= tol * S[~well])
+
+
 def gpu_acc(gpu, b, g, eps, prec):
     db = nb.DeviceBodies(gpu, b)
     grav = nb.GpuNewtonParallelGravity(g, eps * eps, prec)
@@ -75,8 +98,8 @@ def test_examples_all_precisions(gpu, name):
     got = gpu_acc(gpu, b, g, eps, nb.Precision.F64_STRICT)
     for q in range(3):
         assert np.array_equal(got[q], ref[q])
-    assert rel_err(gpu_acc(gpu, b, g, eps, nb.Precision.F64), ref).max() <= 1e-12
-    assert rel_err(gpu_acc(gpu, b, g, eps, nb.Precision.F32C), ref).max() <= 1e-5
+    check_small(gpu_acc(gpu, b, g, eps, nb.Precision.F64), ref, b, g, eps, 1e-12)
+    check_small(gpu_acc(gpu, b, g, eps, nb.Precision.F32C), ref, b, g, eps, 1e-5)
 
 
 @pytest.mark.parametrize("n", [1000, 4096])

@@ -27,6 +27,20 @@ def bounds(off, n, t0, u):
     return [(off, t0, False), (t0, b1, True), (b1, c1, False), (c1, e_up, True)]
 
 
+def clip_part(regions, off, n, u, nsplit, part):
+    """kernels.hip clip_part: restrict regions to one part's chunk run."""
+    if nsplit <= 1:
+        return regions
+    nchunks = (n + u - 1) // u
+    lo = off + (nchunks * part // nsplit) * u
+    hi = off + (nchunks * (part + 1) // nsplit) * u
+    out = []
+    for a, e, m in regions:
+        a, e = max(a, lo), min(e, hi)
+        out.append((a, max(a, e), m))
+    return out
+
+
 def accessed(j0, j1, u, prefetch):
     if j0 >= j1:
         return []
@@ -67,3 +81,23 @@ def test_source_indices_in_bounds_and_ordered(u, prefetch):
 def test_slack_covers_largest_prefetch():
     assert K_PAD >= 8          # one f32 chunk (U = 8) past a rounded-up end
     assert ALIGN % 8 == 0 and K_WAVE % 8 == 0
+
+
+@pytest.mark.parametrize("u", [4, 8])
+def test_parts_cover_sources_once_in_order(u):
+    for n in (513, 1000, 1024, 4096, 8191, 65536 // 16):
+        off = 2 * ALIGN
+        span = (n + ALIGN - 1) // ALIGN * ALIGN
+        tiles = (n + K_WAVE - 1) // K_WAVE
+        for nsplit in (1, 2, 3, 4, 8):
+            for tile in (0, tiles // 2, tiles - 1):
+                t0 = off + tile * K_WAVE
+                seen = []
+                for part in range(nsplit):
+                    for j0, j1, _ in clip_part(bounds(off, n, t0, u), off, n, u,
+                                               nsplit, part):
+                        assert (j0 - off) % u == 0 and (j1 - off) % u == 0
+                        for j in accessed(j0, j1, u, True):
+                            assert j < off + span + K_PAD
+                        seen.extend(range(j0, j1))
+                assert seen == list(range(off, off + (n + u - 1) // u * u))
