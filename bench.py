@@ -128,7 +128,7 @@ def run_single(args, gpu):
         "value": rate32, "unit": "pair-interactions/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms32 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair terms, f64 state",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair terms (f64 differences), f64 state",
         "data": "synthetic (seeded Plummer sphere, generated on the host)",
         "config": {"workload": "single Plummer sphere N=65536, softening 0.005, "
                                "dt 5e-4, velocity-verlet", "n": n, "g": g,
@@ -171,7 +171,7 @@ def run_batched(args, gpu):
         "value": rate32, "unit": "pair-interactions/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms32 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 pair terms, f64 state",
+        "dtype": "f32 pair terms (f64 differences), f64 state",
         "data": "synthetic (seeded star systems)",
         "config": {"workload": "512 star systems, N log-uniform [64, 8192], "
                                "euler/vv/rk4 1:1:1, dt 1e-3", "systems": nsys,
@@ -183,12 +183,132 @@ def run_batched(args, gpu):
     }
 
 
+def run_serve(args, gpu):
+    """Poisson request stream against the continuous-batching server."""
+    import threading
+    rng = np.random.default_rng(11)
+    rates = [float(r) for r in args.rates.split(",")]
+    nreq = args.requests
+    # pre-generate the request stream (host side, not timed)
+    reqs = []
+    for k in range(nreq):
+        n = int(np.exp(rng.uniform(np.log(32), np.log(16384))))
+        b = nb.plummer(n, 50_000 + k) if rng.random() < 0.5 else \
+            nb.star_system(n, 50_000 + k)
+        reqs.append(dict(b=b, integ=nb.IntegratorMethod(int(rng.integers(0, 3))),
+                         dt=float(rng.choice([1e-3, 5e-4])),
+                         steps=int(rng.integers(10, 1001))))
+    results = []
+    for rate in rates:
+        srv = nb.Server(capacity_bodies=4 << 20, max_systems=4096, snapshot_every=10,
+                        precision=nb.Precision.F32C)
+        gaps = rng.exponential(1.0 / rate, nreq)
+        lat, done, interactions = [], [0], 0.0
+        t_first = [None]
+
+        def consume():
+            while done[0] < nreq:
+                r = srv.poll(timeout_ms=2000, capacity=16384)
+                if r is None:
+                    continue
+                ev, _ = r
+                if ev["kind"] in (nb.EVENT_DONE, nb.EVENT_FAILED, nb.EVENT_CANCELLED):
+                    if ev["kind"] != nb.EVENT_DONE:
+                        raise RuntimeError(ev["message"])
+                    lat.append(ev["t_event"] - ev["t_submit"])
+                    done[0] += 1
+
+        th = threading.Thread(target=consume)
+        th.start()
+        t0 = time.perf_counter()
+        t_next = t0
+        for k, q in enumerate(reqs):
+            t_next += gaps[k]
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            srv.submit(q["b"], q["integ"], q["dt"], q["steps"], eps2=1e-4,
+                       want_snapshots=True)
+        th.join(timeout=600)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        lat_ms = np.array(lat) * 1e3
+        res = {"rate_req_s": rate, "requests": nreq, "wall_s": wall,
+               "solves_s": nreq / wall, "p50_ms": float(np.percentile(lat_ms, 50)),
+               "p99_ms": float(np.percentile(lat_ms, 99)),
+               "interactions_s": st["interactions"] / wall,
+               "shared_steps": st["shared_steps"]}
+        log(f"serve rate {rate}/s: {res}")
+        results.append(res)
+    best = max(results, key=lambda r: r["interactions_s"])
+    return {
+        "metric": "served solves/s (p50/p99 latency per rate in `serving`)",
+        "value": best["solves_s"], "unit": "solves/s", "n_gpus": 1,
+        "steps": None, "warmup": 0, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pair terms (f64 differences), f64 state",
+        "data": "synthetic Poisson request stream (Plummer / star systems)",
+        "config": {"workload": "served requests, N log-uniform [32, 16384], steps "
+                               "uniform [10, 1000], random integrator, dt in "
+                               "{1e-3, 5e-4}, snapshots every 10 steps",
+                   "requests_per_rate": nreq, "rates": rates,
+                   "parallelism": "none (1 GPU)"},
+        "roofline": None, "cpu_baseline": None, "serving": results,
+    }
+
+
+def run_large(args, gpu):
+    """Two colliding Plummer spheres, N=1,048,576, RK4 (4 evaluations/step)."""
+    n, g, eps, dt = 1 << 20, 1.0, 0.002, 2.5e-4
+    b = nb.colliding(n, 5, separation=10.0, rel_speed=0.5)
+    out = {}
+    for prec in (nb.Precision.F32C, nb.Precision.F64):
+        grav = nb.GpuNewtonParallelGravity(g, eps * eps, prec)
+        integ = nb.GpuRungeKuttaIntegrator(dt)
+        db = nb.DeviceBodies(gpu, b)
+        e0 = energy(gpu, db, g, eps)
+        gpu.synchronize()
+        gpu.record(0)
+        integ.step(gpu, db, grav, args.steps)
+        gpu.record(1)
+        ms = gpu.elapsed_ms(0, 1)
+        e1 = energy(gpu, db, g, eps)
+        out[prec] = (ms, abs(e1[2] - e0[2]) / abs(e0[2]))
+        log(f"large {prec.name}: {ms:.0f} ms for {args.steps} RK4 steps, "
+            f"energy drift {out[prec][1]:.3e}")
+        db.close()
+    inter = n * (n - 1) * 4 * args.steps
+    ms32, drift32 = out[nb.Precision.F32C]
+    ms64, drift64 = out[nb.Precision.F64]
+    rate32, rate64 = inter / (ms32 / 1e3), inter / (ms64 / 1e3)
+    return {
+        "metric": "pair-interactions/s (f32-compensated forces, f64 state)",
+        "value": rate32, "unit": "pair-interactions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 0, "ms_per_step": ms32 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pair terms (f64 differences), f64 state",
+        "data": "synthetic (two seeded Plummer spheres)",
+        "config": {"workload": "two colliding Plummer spheres N=1048576, softening "
+                               "0.002, dt 2.5e-4, runge-kutta", "n": n,
+                   "parallelism": "none (1 GPU)"},
+        "roofline": roofline(rate32, "f32", ms32 / args.steps / 4),
+        "cpu_baseline": None,
+        "fp64_plain": {"value": rate64, "f32c_speedup": rate32 / rate64,
+                       "roofline": roofline(rate64, "f64", ms64 / args.steps / 4)},
+        "accuracy": {"f32c_energy_drift": drift32, "f64_energy_drift": drift64},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["single", "batched"], default="single")
+    ap.add_argument("--workload", choices=["single", "batched", "serve", "large"],
+                    default="single")
+    ap.add_argument("--rates", default="20,50,80", help="serve: request rates (1/s)")
+    ap.add_argument("--requests", type=int, default=150, help="serve: requests per rate")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
@@ -199,11 +319,12 @@ def main():
               "positions per force evaluation -- not built here)", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = 200 if args.workload == "single" else 50
+        args.steps = {"single": 200, "batched": 50, "serve": 0, "large": 2}[args.workload]
     gpu = nb.GpuContext(0)
     name, cus = gpu.device_info()
     log(f"device {name}, {cus} CUs")
-    rec = run_single(args, gpu) if args.workload == "single" else run_batched(args, gpu)
+    rec = {"single": run_single, "batched": run_batched, "serve": run_serve,
+           "large": run_large}[args.workload](args, gpu)
     rec["device"] = {"arch": name, "compute_units": cus}
     print(json.dumps(rec), flush=True)
 

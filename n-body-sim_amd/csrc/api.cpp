@@ -77,6 +77,8 @@ int mode_of(int precision) {
       return nbody::MODE_F64;
     case NBODY_PRECISION_F32C:
       return nbody::MODE_F32C;
+    case NBODY_PRECISION_F32R:
+      return nbody::MODE_F32R;
   }
   throw Error(NBODY_E_INVALID, "unknown precision");
 }

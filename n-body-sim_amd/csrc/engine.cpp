@@ -31,7 +31,7 @@ Engine::Engine(int device, hipStream_t stream, int mode, int64_t capacity,
       mode_(mode),
       cap_(round_up(std::max<int64_t>(capacity, 1), kAlign)),
       max_sys_(std::max(max_systems, 1)) {
-  if (mode < MODE_F64_STRICT || mode > MODE_F32C)
+  if (mode < MODE_F64_STRICT || mode > MODE_F32R)
     throw Error(NBODY_E_INVALID, "unknown precision");
   alloc_arena();
   hip_check(hipMalloc((void**)&d_sys_, sizeof(SysDesc) * max_sys_),
@@ -52,6 +52,7 @@ Engine::~Engine() {
   hipFree(d_diag_);
   hipFree(d_list_);
   hipFree(d_sys_);
+  hipFree(d_arena_);
   hipFree(arena_mem_);
 }
 
@@ -114,6 +115,12 @@ void Engine::alloc_arena() {
             "hipMemset(arena)");
   if (carve((char*)arena_mem_) != arena_bytes_)
     throw Error(NBODY_E_DEVICE, "arena layout mismatch");
+  // kernels read the arena's pointers from one device-resident copy
+  hip_check(hipMalloc((void**)&d_arena_, sizeof(Arena)), "hipMalloc(arena table)");
+  hip_check(hipMemcpyAsync(d_arena_, &A_, sizeof(Arena), hipMemcpyHostToDevice,
+                           stream_),
+            "hipMemcpy(arena table)");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
 // Host-side guard run on every work list before it is uploaded: each item
@@ -303,7 +310,7 @@ void Engine::set_params(int id, const SystemParams& p) {
 }
 
 void Engine::set_mode(int mode) {
-  if (mode < MODE_F64_STRICT || mode > MODE_F32C)
+  if (mode < MODE_F64_STRICT || mode > MODE_F32R)
     throw Error(NBODY_E_INVALID, "unknown precision");
   if (mode == mode_) return;
   mode_ = mode;
@@ -412,10 +419,10 @@ void Engine::run_round(std::vector<WorkItem>& items,
             "hipMemcpy(work list)");
   const int nf = (int)items.size(), nc = (int)combine.size();
   hipError_t e = potential
-                     ? launch_potential(A_, d_items_, nf, d_sys_, stream_)
-                     : launch_force(mode_, A_, d_items_, nf, d_sys_, stream_);
+                     ? launch_potential(d_arena_, d_items_, nf, d_sys_, stream_)
+                     : launch_force(mode_, d_arena_, d_items_, nf, d_sys_, stream_);
   hip_check(e, "force launch");
-  hip_check(launch_combine(mode_, A_, d_items_ + nf, nc, d_sys_, stream_),
+  hip_check(launch_combine(mode_, d_arena_, d_items_ + nf, nc, d_sys_, stream_),
             "combine launch");
   // the pageable host list must not change before the copy is consumed
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
@@ -437,7 +444,7 @@ void Engine::prepare(const std::vector<int>& ids) {
     set_origin(s, d);
     const bool moved = d.ox != h_sys_[id].ox || d.oy != h_sys_[id].oy ||
                        d.oz != h_sys_[id].oz;
-    if (s.prepared && !(moved && mode_ == MODE_F32C)) continue;
+    if (s.prepared && !(moved && mode_ == MODE_F32R)) continue;
     if (moved) table_dirty_ = true;
     h_sys_[id].ox = d.ox;
     h_sys_[id].oy = d.oy;
@@ -455,7 +462,7 @@ void Engine::prepare(const std::vector<int>& ids) {
                            sizeof(WorkItem) * items.size(),
                            hipMemcpyHostToDevice, stream_),
             "hipMemcpy(work list)");
-  hip_check(launch_prepare(A_, d_items_, (int)items.size(), d_sys_, stream_),
+  hip_check(launch_prepare(d_arena_, d_items_, (int)items.size(), d_sys_, stream_),
             "prepare launch");
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
@@ -545,10 +552,10 @@ void Engine::launch_step(const StepLists& L, int which) {
   for (int r = 0; r < 4; ++r) {
     const int k = which * 8 + r * 2;
     if (L.cnt[k] == 0) continue;
-    hip_check(launch_force(mode_, A_, d_items_ + L.off[k], L.cnt[k], d_sys_,
+    hip_check(launch_force(mode_, d_arena_, d_items_ + L.off[k], L.cnt[k], d_sys_,
                            stream_),
               "force launch");
-    hip_check(launch_combine(mode_, A_, d_items_ + L.off[k + 1], L.cnt[k + 1],
+    hip_check(launch_combine(mode_, d_arena_, d_items_ + L.off[k + 1], L.cnt[k + 1],
                              d_sys_, stream_),
               "combine launch");
   }
@@ -620,7 +627,7 @@ void Engine::diagnostics(const std::vector<int>& ids, double* out) {
   hip_check(hipMemcpyAsync(d_items_, prep.data(), sizeof(WorkItem) * prep.size(),
                            hipMemcpyHostToDevice, stream_),
             "hipMemcpy(work list)");
-  hip_check(launch_prepare(A_, d_items_, (int)prep.size(), d_sys_, stream_),
+  hip_check(launch_prepare(d_arena_, d_items_, (int)prep.size(), d_sys_, stream_),
             "prepare launch");
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   std::vector<WorkItem> combine;
@@ -630,7 +637,7 @@ void Engine::diagnostics(const std::vector<int>& ids, double* out) {
   hip_check(hipMemcpyAsync(d_list_, list.data(), sizeof(int32_t) * list.size(),
                            hipMemcpyHostToDevice, stream_),
             "hipMemcpy(list)");
-  hip_check(launch_diag(A_, d_sys_, d_list_, (int)list.size(), d_diag_,
+  hip_check(launch_diag(d_arena_, d_sys_, d_list_, (int)list.size(), d_diag_,
                         stream_),
             "diag launch");
   hip_check(hipMemcpyAsync(out, d_diag_, sizeof(double) * DIAG_COUNT * ids.size(),

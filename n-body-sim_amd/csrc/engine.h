@@ -133,6 +133,7 @@ class Engine {
   int max_sys_;
   Arena A_{};
   void* arena_mem_ = nullptr;
+  Arena* d_arena_ = nullptr;  // device copy of A_
   size_t arena_bytes_ = 0;
   SysDesc* d_sys_ = nullptr;
   std::vector<SysDesc> h_sys_;

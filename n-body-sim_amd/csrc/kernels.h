@@ -7,21 +7,21 @@
 namespace nbody {
 
 // One force evaluation over a work list, with each item's integrator stage
-// fused into the epilogue.  mode: Mode.  items/sys: device pointers.
-hipError_t launch_force(int mode, const Arena& A, const WorkItem* items,
+// fused into the epilogue.  mode: Mode.  A, items, sys: device pointers.
+hipError_t launch_force(int mode, const Arena* A, const WorkItem* items,
                         int nitems, const SysDesc* sys, hipStream_t s);
 // Sums the partial results of split source ranges in part order, then runs
 // each target's stage epilogue (or stores its potential for STAGE_POT).
-hipError_t launch_combine(int mode, const Arena& A, const WorkItem* items,
+hipError_t launch_combine(int mode, const Arena* A, const WorkItem* items,
                           int nitems, const SysDesc* sys, hipStream_t s);
 // Per-body softened potential sum_j g*m_j / sqrt(r^2 + eps2) into A.phi (f64).
-hipError_t launch_potential(const Arena& A, const WorkItem* items, int nitems,
+hipError_t launch_potential(const Arena* A, const WorkItem* items, int nitems,
                             const SysDesc* sys, hipStream_t s);
 // Snapshot (and gm) rebuild from the master state.
-hipError_t launch_prepare(const Arena& A, const WorkItem* items, int nitems,
+hipError_t launch_prepare(const Arena* A, const WorkItem* items, int nitems,
                           const SysDesc* sys, hipStream_t s);
 // Per-system diagnostics (Diag layout), one workgroup per listed system.
-hipError_t launch_diag(const Arena& A, const SysDesc* sys, const int32_t* list,
+hipError_t launch_diag(const Arena* A, const SysDesc* sys, const int32_t* list,
                        int nlist, double* out, hipStream_t s);
 hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s);
 hipError_t launch_probe(int* out, hipStream_t s);

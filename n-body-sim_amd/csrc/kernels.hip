@@ -14,6 +14,8 @@
 // running launch ever reads a value this launch writes.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 #include "nbody_types.h"
 
@@ -23,8 +25,27 @@ namespace {
 typedef __attribute__((address_space(4))) const double cf64;
 typedef __attribute__((address_space(4))) const float cf32;
 typedef __attribute__((address_space(4))) const int32_t ci32;
+// Arena and system descriptors are read through constant-address-space
+// pointers at the point of use: each field becomes a scalar load where it is
+// needed instead of a kernel-argument load kept live in SGPRs for the whole
+// kernel (which would crowd out the source registers of the hot loops).
+typedef __attribute__((address_space(4))) const Arena CArena;
+typedef __attribute__((address_space(4))) const SysDesc CSys;
 
 constexpr int kWavesPerBlock = kBlock / kWave;
+
+// Source-loop shape: bodies per chunk for the f64 and f32 paths and chunks
+// per compensated group.
+#ifndef NBODY_U64
+#define NBODY_U64 8
+#endif
+#ifndef NBODY_U32
+#define NBODY_U32 8
+#endif
+#ifndef NBODY_KC
+#define NBODY_KC 4
+#endif
+constexpr int kU64 = NBODY_U64, kU32 = NBODY_U32, kKC = NBODY_KC;
 
 // Wave-uniform load of a small descriptor through the constant address space
 // (scalar loads into SGPRs).
@@ -42,20 +63,20 @@ __device__ __forceinline__ T load_uniform(const T* p) {
 // 1/sqrt(x) in binary64: v_rsq_f64 seed refined by one step of the
 // third-order series (1 - e)^(-1/2) = 1 + e/2 + 3e^2/8 + O(e^3) with
 // e = 1 - x*y^2 (Householder iteration of order 3).
+#ifndef NBODY_RSQ_SEED32
+#define NBODY_RSQ_SEED32 0
+#endif
 __device__ __forceinline__ double rsqrt_f64(double x) {
+#if NBODY_RSQ_SEED32
+  // seed from the f32 reciprocal square root (about 2^-23 relative); one
+  // third-order step takes it to below f64 rounding.  Requires x within the
+  // f32 normal range (about 1.2e-38 .. 3.4e38).
+  const double y = (double)__builtin_amdgcn_rsqf((float)x);
+#else
   const double y = __builtin_amdgcn_rsq(x);
+#endif
   const double e = fma(-x * y, y, 1.0);
   return fma(y * e, fma(e, 0.375, 0.5), y);
-}
-
-// Optimisation barrier on a register value (an empty asm statement, no
-// instruction is emitted).  Applied to the double-buffered source registers at
-// the end of a chunk it stops the compiler from re-deriving them as fresh
-// loads at the top of the next chunk, so the next chunk's scalar loads are in
-// flight while the current chunk computes.
-template <typename T>
-__device__ __forceinline__ void keep(T& v) {
-  asm volatile("" : "+s"(v));
 }
 
 struct Bounds {  // chunk-aligned source regions for one work item
@@ -79,29 +100,6 @@ __device__ __forceinline__ Bounds make_bounds(int64_t off, int64_t n,
   b.c1 = max(b.b1, nfull);
   b.d0 = b.c1;
   b.d1 = e_up;
-  return b;
-}
-
-// Restricts the regions to the source range of one part: the system's
-// chunks [0, nchunks) are cut into nsplit contiguous runs, part p taking
-// chunks [p*nchunks/nsplit, (p+1)*nchunks/nsplit).  Parts cover every chunk
-// once, in order.
-template <int U>
-__device__ __forceinline__ Bounds clip_part(Bounds b, int64_t off, int64_t n,
-                                            int nsplit, int part) {
-  if (nsplit <= 1) return b;
-  const int64_t nchunks = (n + U - 1) / U;
-  const int64_t lo = off + (nchunks * part / nsplit) * U;
-  const int64_t hi = off + (nchunks * (part + 1) / nsplit) * U;
-  auto clip = [&](int64_t& a, int64_t& e) {
-    a = max(a, lo);
-    e = min(e, hi);
-    if (e < a) e = a;
-  };
-  clip(b.a0, b.a1);
-  clip(b.b0, b.b1);
-  clip(b.c0, b.c1);
-  clip(b.d0, b.d1);
   return b;
 }
 
@@ -142,51 +140,144 @@ __device__ __forceinline__ void sweep_strict(const cf64* X, const cf64* Y,
   }
 }
 
-// ---- binary64, fast: rsq + refinement, fused multiply-add.  Sources for
-// the next chunk are requested before the current chunk is computed.
-template <bool MASK, bool POT>
-__device__ __forceinline__ void sweep_f64(const cf64* X, const cf64* Y,
-                                          const cf64* Z, const cf64* G,
-                                          int64_t j0, int64_t j1, int64_t i,
-                                          int64_t end, double xi, double yi,
-                                          double zi, double eps2, double& ax,
-                                          double& ay, double& az) {
-  constexpr int U = 4;
-  if (j0 >= j1) return;
-  double cx[U], cy[U], cz[U], cg[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    cx[u] = X[j0 + u];
-    cy[u] = Y[j0 + u];
-    cz[u] = Z[j0 + u];
-    cg[u] = G[j0 + u];
-  }
-  for (int64_t j = j0; j < j1; j += U) {
-    // prefetch: j + 2U - 1 < end_up + U <= cap + kPad, always in bounds
-    double nx[U], ny[U], nz[U], ng[U];
+// ---- fast paths: one loop over the part's source range ------------------
+//
+// The sources [jlo, jhi) of a part are walked in groups of KC chunks of U
+// bodies.  A group that overlaps the wave's own targets [t0, t0+64) or runs
+// past the system's end takes the masked variant (a uniform scalar branch);
+// all other groups run unmasked.  Each chunk's sources arrive through scalar
+// loads; with several waves per SIMD (split source ranges) their latency is
+// covered by the other waves.
+
+// Chunk-aligned source range of one part: the system's chunks [0, nchunks)
+// cut into nsplit contiguous runs, part p taking [p*nchunks/nsplit,
+// (p+1)*nchunks/nsplit).  Parts cover every chunk once, in order.
+template <int U>
+__device__ __forceinline__ void part_range(int64_t off, int64_t n, int nsplit,
+                                           int part, int64_t& lo,
+                                           int64_t& hi) {
+  const uint32_t nchunks = (uint32_t)((n + U - 1) / U);
+  lo = off + (int64_t)((uint64_t)nchunks * (uint32_t)part / (uint32_t)nsplit) * U;
+  hi = off +
+       (int64_t)((uint64_t)nchunks * (uint32_t)(part + 1) / (uint32_t)nsplit) * U;
+}
+
+template <bool M>
+using Masked = std::integral_constant<bool, M>;
+
+// Source bundles: the arrays a chunk reads and its register buffer.
+struct SrcF64 {  // f64 positions and g*m
+  const cf64 *X, *Y, *Z, *G;
+  template <int U>
+  struct Buf {
+    double x[U], y[U], z[U], g[U];
+  };
+  template <int U>
+  __device__ __forceinline__ void load(int64_t j, Buf<U>& b) const {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      nx[u] = X[j + U + u];
-      ny[u] = Y[j + U + u];
-      nz[u] = Z[j + U + u];
-      ng[u] = G[j + U + u];
+      b.x[u] = X[j + u];
+      b.y[u] = Y[j + u];
+      b.z[u] = Z[j + u];
+      b.g[u] = G[j + u];
     }
+  }
+};
+
+struct SrcF32 {  // f32 positions re-centred on the system origin, f32 g*m
+  const cf32 *X, *Y, *Z, *G;
+  template <int U>
+  struct Buf {
+    float x[U], y[U], z[U], g[U];
+  };
+  template <int U>
+  __device__ __forceinline__ void load(int64_t j, Buf<U>& b) const {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      double dx = cx[u] - xi;
-      double dy = cy[u] - yi;
-      double dz = cz[u] - zi;
-      // masked sources contribute exactly zero, whatever values they hold
+      b.x[u] = X[j + u];
+      b.y[u] = Y[j + u];
+      b.z[u] = Z[j + u];
+      b.g[u] = G[j + u];
+    }
+  }
+};
+
+struct SrcX64 {  // f64 positions, f32 g*m
+  const cf64 *X, *Y, *Z;
+  const cf32* G;
+  template <int U>
+  struct Buf {
+    double x[U], y[U], z[U];
+    float g[U];
+  };
+  template <int U>
+  __device__ __forceinline__ void load(int64_t j, Buf<U>& b) const {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      b.x[u] = X[j + u];
+      b.y[u] = Y[j + u];
+      b.z[u] = Z[j + u];
+      b.g[u] = G[j + u];
+    }
+  }
+};
+
+// chunk(masked, buf, j) consumes one chunk; flush() runs after every group.
+// Reads stay below hi <= off + roundup(n, U) <= off + span.
+template <int U, int KC, typename Src, typename Chunk, typename Flush>
+__device__ __forceinline__ void drive(const Src& src, int64_t jlo, int64_t jhi,
+                                      int64_t t0, int64_t end, Chunk&& chunk,
+                                      Flush&& flush) {
+  auto step = [&](int64_t j, auto masked) {
+    typename Src::template Buf<U> b;
+    src.template load<U>(j, b);
+    chunk(masked, b, j);
+  };
+  constexpr int GRP = KC * U;
+  int64_t j = jlo;
+  for (; j + GRP <= jhi; j += GRP) {
+    const bool m = (j < t0 + kWave && j + GRP > t0) || (j + GRP > end);
+    // not unrolled: bounds the live source registers to one chunk
+    if (m) {
+#pragma unroll 1
+      for (int g = 0; g < KC; ++g) step(j + g * U, Masked<true>{});
+    } else {
+#pragma unroll 1
+      for (int g = 0; g < KC; ++g) step(j + g * U, Masked<false>{});
+    }
+    flush();
+  }
+  for (; j < jhi; j += U) {  // remaining chunks, always masked
+    step(j, Masked<true>{});
+    flush();
+  }
+}
+
+// binary64 pair terms: rsq + refinement, fused multiply-add (POT: potential).
+template <bool POT, int U>
+struct ChunkF64 {
+  int64_t i, end;
+  double xi, yi, zi, eps2;
+  double &ax, &ay, &az;
+  template <typename MK>
+  __device__ __forceinline__ void operator()(
+      MK, const SrcF64::Buf<U>& b, int64_t j) const {
+    constexpr bool MASK = MK::value;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      double dx = b.x[u] - xi;
+      double dy = b.y[u] - yi;
+      double dz = b.z[u] - zi;
       const int64_t jj = j + u;
       const bool ok = !MASK || ((jj != i) && (jj < end));
-      if (MASK) {
+      if (MASK) {  // masked sources contribute exactly zero
         dx = ok ? dx : 0.0;
         dy = ok ? dy : 0.0;
         dz = ok ? dz : 0.0;
       }
       const double r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, eps2)));
       const double inv = rsqrt_f64(r2);
-      double k = POT ? cg[u] * inv : cg[u] * ((inv * inv) * inv);
+      double k = POT ? b.g[u] * inv : b.g[u] * ((inv * inv) * inv);
       if (MASK) k = ok ? k : 0.0;
       if (POT) {
         ax += k;
@@ -196,24 +287,13 @@ __device__ __forceinline__ void sweep_f64(const cf64* X, const cf64* Y,
         az = fma(k, dz, az);
       }
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      cx[u] = nx[u];
-      cy[u] = ny[u];
-      cz[u] = nz[u];
-      cg[u] = ng[u];
-      keep(cx[u]);
-      keep(cy[u]);
-      keep(cz[u]);
-      keep(cg[u]);
-    }
   }
-}
+};
 
-// ---- binary32 pair terms, compensated accumulation.  Each chunk of U terms
-// is summed in f32, then added to the running sum with Kahan's compensated
-// summation (Kahan 1965), which keeps the running sum's rounding error at the
-// level of one chunk sum instead of growing with n.
+// ---- binary32 pair terms, compensated accumulation.  Each group of KC
+// chunks is summed in f32, then added to the running sum with Kahan's
+// compensated summation (Kahan 1965), which keeps the running sum's rounding
+// error at the level of one group sum instead of growing with n.
 struct Kahan3 {
   float sx, sy, sz, cx, cy, cz;
 };
@@ -225,82 +305,58 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
   s = t;
 }
 
-template <bool MASK>
-__device__ __forceinline__ void sweep_f32(const cf32* X, const cf32* Y,
-                                          const cf32* Z, const cf32* G,
-                                          int64_t j0, int64_t j1, int64_t i,
-                                          int64_t end, float xi, float yi,
-                                          float zi, float eps2, Kahan3& acc) {
-  constexpr int U = 8;
-  if (j0 >= j1) return;
-  float cx[U], cy[U], cz[U], cg[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    cx[u] = X[j0 + u];
-    cy[u] = Y[j0 + u];
-    cz[u] = Z[j0 + u];
-    cg[u] = G[j0 + u];
-  }
-  for (int64_t j = j0; j < j1; j += U) {
-    float nx[U], ny[U], nz[U], ng[U];
+// Src = SrcX64 (S = double): f64 positions, each difference rounded once
+// to f32 -- exact to f64 rounding at any distance from the origin, so close
+// pairs keep full f32 accuracy.  Src = SrcF32 (S = float): positions of a
+// re-centred f32 snapshot (faster; close pairs far from the origin lose
+// accuracy to the rounding of their absolute positions).
+template <typename Src, typename S, int U>
+struct ChunkF32 {
+  int64_t i, end;
+  S xi, yi, zi;
+  float eps2;
+  float &px, &py, &pz;
+  template <typename MK>
+  __device__ __forceinline__ void operator()(
+      MK, const typename Src::template Buf<U>& b, int64_t j) const {
+    constexpr bool MASK = MK::value;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      nx[u] = X[j + U + u];
-      ny[u] = Y[j + U + u];
-      nz[u] = Z[j + U + u];
-      ng[u] = G[j + U + u];
-    }
-    float px = 0.f, py = 0.f, pz = 0.f;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      float dx = cx[u] - xi;
-      float dy = cy[u] - yi;
-      float dz = cz[u] - zi;
+      float dx = (float)(b.x[u] - xi);
+      float dy = (float)(b.y[u] - yi);
+      float dz = (float)(b.z[u] - zi);
       const int64_t jj = j + u;
       const bool ok = !MASK || ((jj != i) && (jj < end));
-      if (MASK) {
+      if (MASK) {  // masked sources contribute exactly zero
         dx = ok ? dx : 0.f;
         dy = ok ? dy : 0.f;
         dz = ok ? dz : 0.f;
       }
       const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
       const float inv = __builtin_amdgcn_rsqf(r2);
-      float k = cg[u] * ((inv * inv) * inv);
+      float k = b.g[u] * ((inv * inv) * inv);
       if (MASK) k = ok ? k : 0.f;
       px = fmaf(k, dx, px);
       py = fmaf(k, dy, py);
       pz = fmaf(k, dz, pz);
     }
-    kahan_add(acc.sx, acc.cx, px);
-    kahan_add(acc.sy, acc.cy, py);
-    kahan_add(acc.sz, acc.cz, pz);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      cx[u] = nx[u];
-      cy[u] = ny[u];
-      cz[u] = nz[u];
-      cg[u] = ng[u];
-      keep(cx[u]);
-      keep(cy[u]);
-      keep(cz[u]);
-      keep(cg[u]);
-    }
   }
-}
+};
 
-// Writes the next source snapshot of body i (f64 copy, or f32 re-centred).
+// Writes the next source snapshot of body i: f64 positions (f64 modes and
+// MODE_F32C), or f32 positions relative to the system origin (MODE_F32R).
 template <int MODE>
-__device__ __forceinline__ void put_snapshot(const Arena& A, int dst,
-                                             int64_t i, const SysDesc& sd,
+__device__ __forceinline__ void put_snapshot(const CArena* A, int dst,
+                                             int64_t i, const CSys* sd,
                                              double x, double y, double z) {
-  if (MODE == MODE_F32C) {
-    A.fx[dst][i] = (float)(x - sd.ox);
-    A.fy[dst][i] = (float)(y - sd.oy);
-    A.fz[dst][i] = (float)(z - sd.oz);
+  if (MODE == MODE_F32R) {
+    A->fx[dst][i] = (float)(x - sd->ox);
+    A->fy[dst][i] = (float)(y - sd->oy);
+    A->fz[dst][i] = (float)(z - sd->oz);
   } else {
-    A.sx[dst][i] = x;
-    A.sy[dst][i] = y;
-    A.sz[dst][i] = z;
+    A->sx[dst][i] = x;
+    A->sy[dst][i] = y;
+    A->sz[dst][i] = z;
   }
 }
 
@@ -308,64 +364,64 @@ __device__ __forceinline__ void put_snapshot(const Arena& A, int dst,
 // Expression trees follow the reference exactly; contraction is off so the
 // stage arithmetic is bit-identical to the reference for identical inputs.
 template <int MODE>
-__device__ __forceinline__ void epilogue(const Arena& A, const SysDesc& sd,
+__device__ __forceinline__ void epilogue(const CArena* A, const CSys* sd,
                                          int stage, int dst, int64_t i,
                                          double ax, double ay, double az) {
 #pragma clang fp contract(off)
-  const double dt = sd.dt;
+  const double dt = sd->dt;
   switch (stage) {
     case STAGE_ACC: {
-      A.ax[i] = ax;
-      A.ay[i] = ay;
-      A.az[i] = az;
+      A->ax[i] = ax;
+      A->ay[i] = ay;
+      A->az[i] = az;
       break;
     }
     case STAGE_EULER: {  // src/integrators/cpu/euler.rs:69-80
-      const double vx = A.vx[i] + ax * dt;
-      const double vy = A.vy[i] + ay * dt;
-      const double vz = A.vz[i] + az * dt;
-      const double x = A.x[i] + vx * dt;
-      const double y = A.y[i] + vy * dt;
-      const double z = A.z[i] + vz * dt;
-      A.vx[i] = vx;
-      A.vy[i] = vy;
-      A.vz[i] = vz;
-      A.x[i] = x;
-      A.y[i] = y;
-      A.z[i] = z;
+      const double vx = A->vx[i] + ax * dt;
+      const double vy = A->vy[i] + ay * dt;
+      const double vz = A->vz[i] + az * dt;
+      const double x = A->x[i] + vx * dt;
+      const double y = A->y[i] + vy * dt;
+      const double z = A->z[i] + vz * dt;
+      A->vx[i] = vx;
+      A->vy[i] = vy;
+      A->vz[i] = vz;
+      A->x[i] = x;
+      A->y[i] = y;
+      A->z[i] = z;
       put_snapshot<MODE>(A, dst, i, sd, x, y, z);
       break;
     }
     case STAGE_VV_PRIME: {  // velocity_verlet.rs:82-86, first half only
       const double dt2 = dt * dt;
-      A.ax[i] = ax;
-      A.ay[i] = ay;
-      A.az[i] = az;
-      const double x = A.x[i] + ((A.vx[i] * dt) + ((0.5 * ax) * dt2));
-      const double y = A.y[i] + ((A.vy[i] * dt) + ((0.5 * ay) * dt2));
-      const double z = A.z[i] + ((A.vz[i] * dt) + ((0.5 * az) * dt2));
+      A->ax[i] = ax;
+      A->ay[i] = ay;
+      A->az[i] = az;
+      const double x = A->x[i] + ((A->vx[i] * dt) + ((0.5 * ax) * dt2));
+      const double y = A->y[i] + ((A->vy[i] * dt) + ((0.5 * ay) * dt2));
+      const double z = A->z[i] + ((A->vz[i] * dt) + ((0.5 * az) * dt2));
       put_snapshot<MODE>(A, dst, i, sd, x, y, z);
       break;
     }
     case STAGE_VV: {  // velocity_verlet.rs:82-86 then :97-101, a_n carried
       const double dt2 = dt * dt;
-      const double px = A.ax[i], py = A.ay[i], pz = A.az[i];  // a_n
-      double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
-      const double x = A.x[i] + ((vx * dt) + ((0.5 * px) * dt2));
-      const double y = A.y[i] + ((vy * dt) + ((0.5 * py) * dt2));
-      const double z = A.z[i] + ((vz * dt) + ((0.5 * pz) * dt2));
+      const double px = A->ax[i], py = A->ay[i], pz = A->az[i];  // a_n
+      double vx = A->vx[i], vy = A->vy[i], vz = A->vz[i];
+      const double x = A->x[i] + ((vx * dt) + ((0.5 * px) * dt2));
+      const double y = A->y[i] + ((vy * dt) + ((0.5 * py) * dt2));
+      const double z = A->z[i] + ((vz * dt) + ((0.5 * pz) * dt2));
       vx = vx + ((0.5 * (px + ax)) * dt);
       vy = vy + ((0.5 * (py + ay)) * dt);
       vz = vz + ((0.5 * (pz + az)) * dt);
-      A.x[i] = x;
-      A.y[i] = y;
-      A.z[i] = z;
-      A.vx[i] = vx;
-      A.vy[i] = vy;
-      A.vz[i] = vz;
-      A.ax[i] = ax;
-      A.ay[i] = ay;
-      A.az[i] = az;
+      A->x[i] = x;
+      A->y[i] = y;
+      A->z[i] = z;
+      A->vx[i] = vx;
+      A->vy[i] = vy;
+      A->vz[i] = vz;
+      A->ax[i] = ax;
+      A->ay[i] = ay;
+      A->az[i] = az;
       put_snapshot<MODE>(A, dst, i, sd, x + ((vx * dt) + ((0.5 * ax) * dt2)),
                          y + ((vy * dt) + ((0.5 * ay) * dt2)),
                          z + ((vz * dt) + ((0.5 * az) * dt2)));
@@ -376,63 +432,63 @@ __device__ __forceinline__ void epilogue(const Arena& A, const SysDesc& sd,
     // the next stage's position slope k_{s+1,r}.
     case STAGE_RK0: {
       const double h = dt / 2.0;
-      const double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
+      const double vx = A->vx[i], vy = A->vy[i], vz = A->vz[i];
       const double k2x = vx + h * ax, k2y = vy + h * ay, k2z = vz + h * az;
-      A.rk[0][i] = vx + (2.0 * k2x);
-      A.rk[1][i] = vy + (2.0 * k2y);
-      A.rk[2][i] = vz + (2.0 * k2z);
-      A.rk[3][i] = ax;
-      A.rk[4][i] = ay;
-      A.rk[5][i] = az;
-      A.rk[6][i] = k2x;
-      A.rk[7][i] = k2y;
-      A.rk[8][i] = k2z;
-      put_snapshot<MODE>(A, dst, i, sd, A.x[i] + h * vx, A.y[i] + h * vy,
-                         A.z[i] + h * vz);
+      A->rk[0][i] = vx + (2.0 * k2x);
+      A->rk[1][i] = vy + (2.0 * k2y);
+      A->rk[2][i] = vz + (2.0 * k2z);
+      A->rk[3][i] = ax;
+      A->rk[4][i] = ay;
+      A->rk[5][i] = az;
+      A->rk[6][i] = k2x;
+      A->rk[7][i] = k2y;
+      A->rk[8][i] = k2z;
+      put_snapshot<MODE>(A, dst, i, sd, A->x[i] + h * vx, A->y[i] + h * vy,
+                         A->z[i] + h * vz);
       break;
     }
     case STAGE_RK1: {
       const double h = dt / 2.0;
-      const double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
+      const double vx = A->vx[i], vy = A->vy[i], vz = A->vz[i];
       const double k3x = vx + h * ax, k3y = vy + h * ay, k3z = vz + h * az;
-      const double k2x = A.rk[6][i], k2y = A.rk[7][i], k2z = A.rk[8][i];
-      A.rk[0][i] = A.rk[0][i] + (2.0 * k3x);
-      A.rk[1][i] = A.rk[1][i] + (2.0 * k3y);
-      A.rk[2][i] = A.rk[2][i] + (2.0 * k3z);
-      A.rk[3][i] = A.rk[3][i] + (2.0 * ax);
-      A.rk[4][i] = A.rk[4][i] + (2.0 * ay);
-      A.rk[5][i] = A.rk[5][i] + (2.0 * az);
-      A.rk[6][i] = k3x;
-      A.rk[7][i] = k3y;
-      A.rk[8][i] = k3z;
-      put_snapshot<MODE>(A, dst, i, sd, A.x[i] + h * k2x, A.y[i] + h * k2y,
-                         A.z[i] + h * k2z);
+      const double k2x = A->rk[6][i], k2y = A->rk[7][i], k2z = A->rk[8][i];
+      A->rk[0][i] = A->rk[0][i] + (2.0 * k3x);
+      A->rk[1][i] = A->rk[1][i] + (2.0 * k3y);
+      A->rk[2][i] = A->rk[2][i] + (2.0 * k3z);
+      A->rk[3][i] = A->rk[3][i] + (2.0 * ax);
+      A->rk[4][i] = A->rk[4][i] + (2.0 * ay);
+      A->rk[5][i] = A->rk[5][i] + (2.0 * az);
+      A->rk[6][i] = k3x;
+      A->rk[7][i] = k3y;
+      A->rk[8][i] = k3z;
+      put_snapshot<MODE>(A, dst, i, sd, A->x[i] + h * k2x, A->y[i] + h * k2y,
+                         A->z[i] + h * k2z);
       break;
     }
     case STAGE_RK2: {
-      const double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
-      const double k3x = A.rk[6][i], k3y = A.rk[7][i], k3z = A.rk[8][i];
-      A.rk[0][i] = A.rk[0][i] + (vx + dt * ax);
-      A.rk[1][i] = A.rk[1][i] + (vy + dt * ay);
-      A.rk[2][i] = A.rk[2][i] + (vz + dt * az);
-      A.rk[3][i] = A.rk[3][i] + (2.0 * ax);
-      A.rk[4][i] = A.rk[4][i] + (2.0 * ay);
-      A.rk[5][i] = A.rk[5][i] + (2.0 * az);
-      put_snapshot<MODE>(A, dst, i, sd, A.x[i] + dt * k3x, A.y[i] + dt * k3y,
-                         A.z[i] + dt * k3z);
+      const double vx = A->vx[i], vy = A->vy[i], vz = A->vz[i];
+      const double k3x = A->rk[6][i], k3y = A->rk[7][i], k3z = A->rk[8][i];
+      A->rk[0][i] = A->rk[0][i] + (vx + dt * ax);
+      A->rk[1][i] = A->rk[1][i] + (vy + dt * ay);
+      A->rk[2][i] = A->rk[2][i] + (vz + dt * az);
+      A->rk[3][i] = A->rk[3][i] + (2.0 * ax);
+      A->rk[4][i] = A->rk[4][i] + (2.0 * ay);
+      A->rk[5][i] = A->rk[5][i] + (2.0 * az);
+      put_snapshot<MODE>(A, dst, i, sd, A->x[i] + dt * k3x, A->y[i] + dt * k3y,
+                         A->z[i] + dt * k3z);
       break;
     }
     case STAGE_RK3: {
       const double s6 = dt / 6.0;
-      const double x = A.x[i] + s6 * A.rk[0][i];
-      const double y = A.y[i] + s6 * A.rk[1][i];
-      const double z = A.z[i] + s6 * A.rk[2][i];
-      A.vx[i] = A.vx[i] + s6 * (A.rk[3][i] + ax);
-      A.vy[i] = A.vy[i] + s6 * (A.rk[4][i] + ay);
-      A.vz[i] = A.vz[i] + s6 * (A.rk[5][i] + az);
-      A.x[i] = x;
-      A.y[i] = y;
-      A.z[i] = z;
+      const double x = A->x[i] + s6 * A->rk[0][i];
+      const double y = A->y[i] + s6 * A->rk[1][i];
+      const double z = A->z[i] + s6 * A->rk[2][i];
+      A->vx[i] = A->vx[i] + s6 * (A->rk[3][i] + ax);
+      A->vy[i] = A->vy[i] + s6 * (A->rk[4][i] + ay);
+      A->vz[i] = A->vz[i] + s6 * (A->rk[5][i] + az);
+      A->x[i] = x;
+      A->y[i] = y;
+      A->z[i] = z;
       put_snapshot<MODE>(A, dst, i, sd, x, y, z);
       break;
     }
@@ -442,7 +498,7 @@ __device__ __forceinline__ void epilogue(const Arena& A, const SysDesc& sd,
 }
 
 template <int MODE, bool POT>
-__global__ __launch_bounds__(kBlock) void k_force(Arena A,
+__global__ __launch_bounds__(kBlock) void k_force(const Arena* Ap,
                                                   const WorkItem* items,
                                                   int nitems,
                                                   const SysDesc* sysd) {
@@ -450,8 +506,9 @@ __global__ __launch_bounds__(kBlock) void k_force(Arena A,
   const int idx = (int)blockIdx.x * kWavesPerBlock + wave;
   if (idx >= nitems) return;
   const WorkItem it = load_uniform(items + idx);
-  const SysDesc sd = load_uniform(sysd + it.sys);
-  const int64_t off = sd.off, n = sd.n, end = off + n;
+  const CArena* A = (const CArena*)Ap;
+  const CSys* sd = (const CSys*)(sysd + it.sys);
+  const int64_t off = sd->off, n = sd->n, end = off + n;
   const int64_t t0 = off + (int64_t)it.tile * kWave;
   const int64_t i = t0 + (int64_t)(threadIdx.x & (kWave - 1));
   const bool active = i < end;
@@ -459,33 +516,51 @@ __global__ __launch_bounds__(kBlock) void k_force(Arena A,
   const int src = it.src;
   double ax = 0.0, ay = 0.0, az = 0.0;
 
-  if (MODE == MODE_F32C) {
-    const cf32* X = (const cf32*)A.fx[src];
-    const cf32* Y = (const cf32*)A.fy[src];
-    const cf32* Z = (const cf32*)A.fz[src];
-    const cf32* G = (const cf32*)A.gm32;
-    const float xi = A.fx[src][il], yi = A.fy[src][il], zi = A.fz[src][il];
-    const float e2 = sd.eps2f;
+  const int nsplit = sd->nsplit, part = it.part;
+  if (MODE == MODE_F32C || MODE == MODE_F32R) {
+    constexpr int U = kU32, KC = kKC;
+    using Src = typename std::conditional<MODE == MODE_F32C, SrcX64,
+                                          SrcF32>::type;
+    using S = typename std::conditional<MODE == MODE_F32C, double,
+                                        float>::type;
+    Src src_bundle;
+    S xi, yi, zi;
+    if constexpr (MODE == MODE_F32C) {
+      src_bundle = SrcX64{(const cf64*)A->sx[src], (const cf64*)A->sy[src],
+                          (const cf64*)A->sz[src], (const cf32*)A->gm32};
+      xi = A->sx[src][il];
+      yi = A->sy[src][il];
+      zi = A->sz[src][il];
+    } else {
+      src_bundle = SrcF32{(const cf32*)A->fx[src], (const cf32*)A->fy[src],
+                          (const cf32*)A->fz[src], (const cf32*)A->gm32};
+      xi = A->fx[src][il];
+      yi = A->fy[src][il];
+      zi = A->fz[src][il];
+    }
     Kahan3 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const Bounds b =
-        clip_part<8>(make_bounds<8>(off, n, t0), off, n, sd.nsplit, it.part);
-    sweep_f32<false>(X, Y, Z, G, b.a0, b.a1, i, end, xi, yi, zi, e2, acc);
-    sweep_f32<true>(X, Y, Z, G, b.b0, b.b1, i, end, xi, yi, zi, e2, acc);
-    sweep_f32<false>(X, Y, Z, G, b.c0, b.c1, i, end, xi, yi, zi, e2, acc);
-    sweep_f32<true>(X, Y, Z, G, b.d0, b.d1, i, end, xi, yi, zi, e2, acc);
+    float px = 0.f, py = 0.f, pz = 0.f;
+    int64_t jlo, jhi;
+    part_range<U>(off, n, nsplit, part, jlo, jhi);
+    ChunkF32<Src, S, U> ch{i, end, xi, yi, zi, sd->eps2f, px, py, pz};
+    drive<U, KC>(src_bundle, jlo, jhi, t0, end, ch, [&] {
+      kahan_add(acc.sx, acc.cx, px);
+      kahan_add(acc.sy, acc.cy, py);
+      kahan_add(acc.sz, acc.cz, pz);
+      px = py = pz = 0.f;
+    });
     ax = (double)acc.sx - (double)acc.cx;
     ay = (double)acc.sy - (double)acc.cy;
     az = (double)acc.sz - (double)acc.cz;
   } else {
-    const cf64* X = (const cf64*)A.sx[src];
-    const cf64* Y = (const cf64*)A.sy[src];
-    const cf64* Z = (const cf64*)A.sz[src];
-    const cf64* G = (const cf64*)A.gm64;
-    const double xi = A.sx[src][il], yi = A.sy[src][il], zi = A.sz[src][il];
-    const double e2 = sd.eps2;
-    const Bounds b =
-        clip_part<4>(make_bounds<4>(off, n, t0), off, n, sd.nsplit, it.part);
+    const cf64* X = (const cf64*)A->sx[src];
+    const cf64* Y = (const cf64*)A->sy[src];
+    const cf64* Z = (const cf64*)A->sz[src];
+    const cf64* G = (const cf64*)A->gm64;
+    const double xi = A->sx[src][il], yi = A->sy[src][il], zi = A->sz[src][il];
+    const double e2 = sd->eps2;
     if (MODE == MODE_F64_STRICT) {
+      const Bounds b = make_bounds<4>(off, n, t0);
       sweep_strict<false>(X, Y, Z, G, b.a0, b.a1, i, end, xi, yi, zi, e2, ax,
                           ay, az);
       sweep_strict<true>(X, Y, Z, G, b.b0, b.b1, i, end, xi, yi, zi, e2, ax,
@@ -495,28 +570,25 @@ __global__ __launch_bounds__(kBlock) void k_force(Arena A,
       sweep_strict<true>(X, Y, Z, G, b.d0, b.d1, i, end, xi, yi, zi, e2, ax,
                          ay, az);
     } else {
-      sweep_f64<false, POT>(X, Y, Z, G, b.a0, b.a1, i, end, xi, yi, zi, e2,
-                            ax, ay, az);
-      sweep_f64<true, POT>(X, Y, Z, G, b.b0, b.b1, i, end, xi, yi, zi, e2,
-                           ax, ay, az);
-      sweep_f64<false, POT>(X, Y, Z, G, b.c0, b.c1, i, end, xi, yi, zi, e2,
-                            ax, ay, az);
-      sweep_f64<true, POT>(X, Y, Z, G, b.d0, b.d1, i, end, xi, yi, zi, e2,
-                           ax, ay, az);
+      constexpr int U = kU64;
+      int64_t jlo, jhi;
+      part_range<U>(off, n, nsplit, part, jlo, jhi);
+      ChunkF64<POT, U> ch{i, end, xi, yi, zi, e2, ax, ay, az};
+      drive<U, kKC>(SrcF64{X, Y, Z, G}, jlo, jhi, t0, end, ch, [] {});
     }
   }
   if (!active) return;
-  if (sd.nsplit > 1) {  // partial sum; k_combine finishes the target
-    const int64_t q = (int64_t)it.part * A.pstride + i;
-    A.px[q] = ax;
+  if (sd->nsplit > 1) {  // partial sum; k_combine finishes the target
+    const int64_t q = (int64_t)it.part * A->pstride + i;
+    A->px[q] = ax;
     if (!POT) {
-      A.py[q] = ay;
-      A.pz[q] = az;
+      A->py[q] = ay;
+      A->pz[q] = az;
     }
     return;
   }
   if (POT) {
-    A.phi[i] = ax;
+    A->phi[i] = ax;
     return;
   }
   epilogue<MODE>(A, sd, it.stage, src ^ 1, i, ax, ay, az);
@@ -525,7 +597,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Arena A,
 // Adds the parts' partial sums of each target in part order (a fixed order,
 // so results are reproducible) and runs the target's stage epilogue.
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_combine(Arena A,
+__global__ __launch_bounds__(kBlock) void k_combine(const Arena* Ap,
                                                     const WorkItem* items,
                                                     int nitems,
                                                     const SysDesc* sysd) {
@@ -533,23 +605,24 @@ __global__ __launch_bounds__(kBlock) void k_combine(Arena A,
   const int idx = (int)blockIdx.x * kWavesPerBlock + wave;
   if (idx >= nitems) return;
   const WorkItem it = load_uniform(items + idx);
-  const SysDesc sd = load_uniform(sysd + it.sys);
-  const int64_t end = sd.off + sd.n;
-  const int64_t i = sd.off + (int64_t)it.tile * kWave +
+  const CArena* A = (const CArena*)Ap;
+  const CSys* sd = (const CSys*)(sysd + it.sys);
+  const int64_t end = sd->off + sd->n;
+  const int64_t i = sd->off + (int64_t)it.tile * kWave +
                     (int64_t)(threadIdx.x & (kWave - 1));
   if (i >= end) return;
-  const int64_t st = A.pstride;
-  double ax = A.px[i];
+  const int64_t st = A->pstride;
+  double ax = A->px[i];
   if (it.stage == STAGE_POT) {
-    for (int p = 1; p < sd.nsplit; ++p) ax = ax + A.px[p * st + i];
-    A.phi[i] = ax;
+    for (int p = 1; p < sd->nsplit; ++p) ax = ax + A->px[p * st + i];
+    A->phi[i] = ax;
     return;
   }
-  double ay = A.py[i], az = A.pz[i];
-  for (int p = 1; p < sd.nsplit; ++p) {
-    ax = ax + A.px[p * st + i];
-    ay = ay + A.py[p * st + i];
-    az = az + A.pz[p * st + i];
+  double ay = A->py[i], az = A->pz[i];
+  for (int p = 1; p < sd->nsplit; ++p) {
+    ax = ax + A->px[p * st + i];
+    ay = ay + A->py[p * st + i];
+    az = az + A->pz[p * st + i];
   }
   epilogue<MODE>(A, sd, it.stage, it.src ^ 1, i, ax, ay, az);
 }
@@ -558,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(Arena A,
 // stage 0 -> the positions themselves; STAGE_VV_PRIME -> the speculative
 // next positions x + ((v*dt) + ((0.5*a)*dt^2)) from the carried a.  Also
 // refreshes gm = g*m in f64 and f32.
-__global__ __launch_bounds__(kBlock) void k_prepare(Arena A,
+__global__ __launch_bounds__(kBlock) void k_prepare(const Arena* Ap,
                                                     const WorkItem* items,
                                                     int nitems,
                                                     const SysDesc* sysd) {
@@ -567,55 +640,57 @@ __global__ __launch_bounds__(kBlock) void k_prepare(Arena A,
   const int idx = (int)blockIdx.x * kWavesPerBlock + wave;
   if (idx >= nitems) return;
   const WorkItem it = load_uniform(items + idx);
-  const SysDesc sd = load_uniform(sysd + it.sys);
-  const int64_t end = sd.off + sd.n;
-  const int64_t i = sd.off + (int64_t)it.tile * kWave +
+  const CArena* A = (const CArena*)Ap;
+  const CSys* sd = (const CSys*)(sysd + it.sys);
+  const int64_t end = sd->off + sd->n;
+  const int64_t i = sd->off + (int64_t)it.tile * kWave +
                     (int64_t)(threadIdx.x & (kWave - 1));
   if (i >= end) return;
-  const double gm = sd.g * A.m[i];
-  A.gm64[i] = gm;
-  A.gm32[i] = (float)gm;
-  double x = A.x[i], y = A.y[i], z = A.z[i];
+  const double gm = sd->g * A->m[i];
+  A->gm64[i] = gm;
+  A->gm32[i] = (float)gm;
+  double x = A->x[i], y = A->y[i], z = A->z[i];
   if (it.stage == STAGE_VV_PRIME) {
-    const double dt = sd.dt, dt2 = dt * dt;
-    x = x + ((A.vx[i] * dt) + ((0.5 * A.ax[i]) * dt2));
-    y = y + ((A.vy[i] * dt) + ((0.5 * A.ay[i]) * dt2));
-    z = z + ((A.vz[i] * dt) + ((0.5 * A.az[i]) * dt2));
+    const double dt = sd->dt, dt2 = dt * dt;
+    x = x + ((A->vx[i] * dt) + ((0.5 * A->ax[i]) * dt2));
+    y = y + ((A->vy[i] * dt) + ((0.5 * A->ay[i]) * dt2));
+    z = z + ((A->vz[i] * dt) + ((0.5 * A->az[i]) * dt2));
   }
   const int d = it.src;
-  A.sx[d][i] = x;
-  A.sy[d][i] = y;
-  A.sz[d][i] = z;
-  A.fx[d][i] = (float)(x - sd.ox);
-  A.fy[d][i] = (float)(y - sd.oy);
-  A.fz[d][i] = (float)(z - sd.oz);
+  A->sx[d][i] = x;
+  A->sy[d][i] = y;
+  A->sz[d][i] = z;
+  A->fx[d][i] = (float)(x - sd->ox);
+  A->fy[d][i] = (float)(y - sd->oy);
+  A->fz[d][i] = (float)(z - sd->oz);
 }
 
 // Per-system f64 diagnostics, one workgroup per listed system, fixed
 // summation order (thread t takes bodies t, t+256, ...; then a fixed LDS
 // tree), so results are reproducible bit for bit.  Needs phi from a
 // potential pass.  out[s*9 + DIAG_*].
-__global__ __launch_bounds__(kBlock) void k_diag(Arena A, const SysDesc* sysd,
+__global__ __launch_bounds__(kBlock) void k_diag(const Arena* Ap, const SysDesc* sysd,
                                                  const int32_t* list,
                                                  double* out) {
   __shared__ double red[DIAG_COUNT][kBlock];
   const int s = list[blockIdx.x];
-  const SysDesc sd = sysd[s];
+  const CArena* A = (const CArena*)Ap;
+  const CSys* sd = (const CSys*)(sysd + s);
   double acc[DIAG_COUNT];
 #pragma unroll
   for (int q = 0; q < DIAG_COUNT; ++q) acc[q] = 0.0;
-  for (int64_t k = threadIdx.x; k < sd.n; k += kBlock) {
-    const int64_t i = sd.off + k;
-    const double m = A.m[i];
-    const double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
+  for (int64_t k = threadIdx.x; k < sd->n; k += kBlock) {
+    const int64_t i = sd->off + k;
+    const double m = A->m[i];
+    const double vx = A->vx[i], vy = A->vy[i], vz = A->vz[i];
     acc[DIAG_K] += 0.5 * m * ((vx * vx + vy * vy) + vz * vz);
-    acc[DIAG_U] += m * A.phi[i];
+    acc[DIAG_U] += m * A->phi[i];
     acc[DIAG_PX] += m * vx;
     acc[DIAG_PY] += m * vy;
     acc[DIAG_PZ] += m * vz;
-    acc[DIAG_MX] += m * A.x[i];
-    acc[DIAG_MY] += m * A.y[i];
-    acc[DIAG_MZ] += m * A.z[i];
+    acc[DIAG_MX] += m * A->x[i];
+    acc[DIAG_MY] += m * A->y[i];
+    acc[DIAG_MZ] += m * A->z[i];
     acc[DIAG_M] += m;
   }
 #pragma unroll
@@ -654,7 +729,7 @@ inline unsigned blocks_for(int nitems) {
 
 }  // namespace
 
-hipError_t launch_force(int mode, const Arena& A, const WorkItem* items,
+hipError_t launch_force(int mode, const Arena* A, const WorkItem* items,
                         int nitems, const SysDesc* sys, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   const dim3 grid(blocks_for(nitems)), block(kBlock);
@@ -667,6 +742,10 @@ hipError_t launch_force(int mode, const Arena& A, const WorkItem* items,
       hipLaunchKernelGGL((k_force<MODE_F64, false>), grid, block, 0, s, A,
                          items, nitems, sys);
       break;
+    case MODE_F32R:
+      hipLaunchKernelGGL((k_force<MODE_F32R, false>), grid, block, 0, s, A,
+                         items, nitems, sys);
+      break;
     case MODE_F32C:
       hipLaunchKernelGGL((k_force<MODE_F32C, false>), grid, block, 0, s, A,
                          items, nitems, sys);
@@ -677,7 +756,7 @@ hipError_t launch_force(int mode, const Arena& A, const WorkItem* items,
   return hipGetLastError();
 }
 
-hipError_t launch_combine(int mode, const Arena& A, const WorkItem* items,
+hipError_t launch_combine(int mode, const Arena* A, const WorkItem* items,
                           int nitems, const SysDesc* sys, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   const dim3 grid(blocks_for(nitems)), block(kBlock);
@@ -690,6 +769,10 @@ hipError_t launch_combine(int mode, const Arena& A, const WorkItem* items,
       hipLaunchKernelGGL(k_combine<MODE_F64>, grid, block, 0, s, A, items,
                          nitems, sys);
       break;
+    case MODE_F32R:
+      hipLaunchKernelGGL(k_combine<MODE_F32R>, grid, block, 0, s, A, items,
+                         nitems, sys);
+      break;
     case MODE_F32C:
       hipLaunchKernelGGL(k_combine<MODE_F32C>, grid, block, 0, s, A, items,
                          nitems, sys);
@@ -700,7 +783,7 @@ hipError_t launch_combine(int mode, const Arena& A, const WorkItem* items,
   return hipGetLastError();
 }
 
-hipError_t launch_potential(const Arena& A, const WorkItem* items, int nitems,
+hipError_t launch_potential(const Arena* A, const WorkItem* items, int nitems,
                             const SysDesc* sys, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   hipLaunchKernelGGL((k_force<MODE_F64, true>), dim3(blocks_for(nitems)),
@@ -708,7 +791,7 @@ hipError_t launch_potential(const Arena& A, const WorkItem* items, int nitems,
   return hipGetLastError();
 }
 
-hipError_t launch_prepare(const Arena& A, const WorkItem* items, int nitems,
+hipError_t launch_prepare(const Arena* A, const WorkItem* items, int nitems,
                           const SysDesc* sys, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_prepare, dim3(blocks_for(nitems)), dim3(kBlock), 0, s,
@@ -716,7 +799,7 @@ hipError_t launch_prepare(const Arena& A, const WorkItem* items, int nitems,
   return hipGetLastError();
 }
 
-hipError_t launch_diag(const Arena& A, const SysDesc* sys, const int32_t* list,
+hipError_t launch_diag(const Arena* A, const SysDesc* sys, const int32_t* list,
                        int nlist, double* out, hipStream_t s) {
   if (nlist <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_diag, dim3(nlist), dim3(kBlock), 0, s, A, sys, list,

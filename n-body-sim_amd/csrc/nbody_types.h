@@ -4,7 +4,8 @@
 //   m, gm64 (= G*m), gm32         bodies' masses, G folded in once per system
 //   x, y, z, vx, vy, vz           master state, binary64
 //   sx/sy/sz[2]                   f64 source snapshot, double-buffered
-//   fx/fy/fz[2]                   f32 source snapshot (re-centred), double-buffered
+//   fx/fy/fz[2]                   f32 source snapshot re-centred on the centre of
+//                                 mass (MODE_F32R), double-buffered
 //   ax, ay, az                    carried acceleration (velocity Verlet) or the
 //                                 output of a plain force evaluation
 //   rk[9]                         RK4 running sums and next-stage velocity
@@ -27,8 +28,11 @@ enum Mode : int32_t {            // force arithmetic (public NBODY_PRECISION_*)
   MODE_F64_STRICT = 0,           // reference operation order, no contraction,
                                  // correctly rounded sqrt and divide
   MODE_F64 = 1,                  // f64, rsq + refinement, fused multiply-add
-  MODE_F32C = 2,                 // f32 pair terms on re-centred snapshots,
-                                 // compensated accumulation
+  MODE_F32C = 2,                 // f64 positions, each difference rounded
+                                 // once to f32, f32 pair terms, compensated
+                                 // accumulation
+  MODE_F32R = 3,                 // f32 positions re-centred on the centre of
+                                 // mass, f32 pair terms, compensated
 };
 
 enum Integ : int32_t { INTEG_EULER = 0, INTEG_VV = 1, INTEG_RK4 = 2 };
@@ -75,7 +79,7 @@ struct Arena {         // device pointers, passed by value to every kernel
   float *gm32;
   double *x, *y, *z, *vx, *vy, *vz;
   double *sx[2], *sy[2], *sz[2];
-  float *fx[2], *fy[2], *fz[2];
+  float *fx[2], *fy[2], *fz[2];     // f32 snapshot: x - origin (MODE_F32R)
   double *ax, *ay, *az;
   double *rk[9];
   double *phi;

@@ -74,6 +74,8 @@ int mode_of(int precision) {
       return MODE_F64;
     case NBODY_PRECISION_F32C:
       return MODE_F32C;
+    case NBODY_PRECISION_F32R:
+      return MODE_F32R;
   }
   throw Error(NBODY_E_INVALID, "unknown precision");
 }

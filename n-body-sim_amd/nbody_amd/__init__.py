@@ -26,7 +26,9 @@ from dataclasses import dataclass
 import numpy as np
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG_DIR, "libnbody_amd.so")
+# NBODY_AMD_LIB selects another build of the same library (e.g. an A/B build
+# of a kernel variant); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get("NBODY_AMD_LIB") or os.path.join(_PKG_DIR, "libnbody_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_PKG_DIR), "include", "nbody.h")
 
 OK = 0
@@ -137,9 +139,10 @@ def _f64(a) -> np.ndarray:
 
 # ---- reference selection enums (src/args.rs:84-187) -----------------------
 class Precision(enum.IntEnum):
-    F64_STRICT = 0
-    F64 = 1
-    F32C = 2
+    F64_STRICT = 0   # reference operation order: bit-identical to the f64 CPU path
+    F64 = 1          # f64, rsq + refinement, FMA
+    F32C = 2         # f64 differences rounded to f32, f32 terms, compensated sums
+    F32R = 3         # f32 positions re-centred on the centre of mass (faster)
 
 
 class GravityMethod(enum.IntEnum):
