@@ -88,10 +88,28 @@ def energy(gpu, db, g, eps):
     return out
 
 
-def roofline(rate, dtype, kernel_ms):
+def measured_traffic():
+    """HBM bytes per force launch of the N=65536 workload, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (counters scaled by a copy-kernel
+    calibration).  None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "force_single_n65536_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("machine-readable: traffic_bytes_per_launch"):
+                kv = dict(t.split("=") for t in line.split()[2:])
+                return {k: int(v) for k, v in kv.items()}
+    except OSError:
+        pass
+    return None
+
+
+def roofline(rate, dtype, kernel_ms, traffic=None):
+    """VALU roofline of the force kernel: achieved = interactions/s x 20 flops
+    (the counting convention of the reference's published figures) against
+    MI355X's vector peak for the arithmetic type."""
     ach = rate * FLOPS_PER_INTERACTION / 1e12
     return {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS[dtype],
-            "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dtype], "traffic": None,
+            "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dtype], "traffic": traffic,
             "kernel_avg_ms": kernel_ms}
 
 
@@ -105,6 +123,7 @@ def run_single(args, gpu):
         ms, acc0, e0, e1 = time_single(gpu, b, prec, g, eps, dt, args.steps, args.warmup)
         res[prec] = (ms, acc0, e0, e1)
         log(f"  {ms:.1f} ms -> {inter / (ms / 1e3):.4e} int/s")
+    traffic = measured_traffic()
     ms32, acc32, e0_32, e1_32 = res[nb.Precision.F32C]
     ms64, acc64, e0_64, e1_64 = res[nb.Precision.F64]
     rate32 = inter / (ms32 / 1e3)
@@ -134,10 +153,12 @@ def run_single(args, gpu):
                                "dt 5e-4, velocity-verlet", "n": n, "g": g,
                    "softening": eps, "dt": dt, "integrator": "velocity-verlet",
                    "global_batch": 1, "parallelism": "none (1 GPU)"},
-        "roofline": roofline(rate32, "f32", ms32 / args.steps),
+        "roofline": roofline(rate32, "f32", ms32 / args.steps,
+                             (traffic or {}).get("f32c")),
         "cpu_baseline": cpu,
         "fp64_plain": {"value": rate64, "ms_per_step": ms64 / args.steps,
-                       "roofline": roofline(rate64, "f64", ms64 / args.steps),
+                       "roofline": roofline(rate64, "f64", ms64 / args.steps,
+                                            (traffic or {}).get("f64")),
                        "f32c_speedup": rate32 / rate64},
         "accuracy": acc_rep,
     }
@@ -300,12 +321,38 @@ def run_large(args, gpu):
     }
 
 
+def run_copy(args, gpu):
+    """Counter calibration: one device-to-device copy of exactly 256 MiB
+    (HIP runtime blit).  Under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE its
+    kernel must read and write 262144 KiB; the ratio measured/expected scales
+    the counters of the force kernels into HBM bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = 256 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0:  # device to device
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"metric": "copy calibration", "value": nbytes, "unit": "bytes",
+            "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": None,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u8", "data": "synthetic", "config": {"workload": "copy 256 MiB"},
+            "roofline": None, "cpu_baseline": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["single", "batched", "serve", "large"],
+    ap.add_argument("--workload", choices=["single", "batched", "serve", "large", "copy"],
                     default="single")
     ap.add_argument("--rates", default="20,50,80", help="serve: request rates (1/s)")
     ap.add_argument("--requests", type=int, default=150, help="serve: requests per rate")
@@ -319,12 +366,13 @@ def main():
               "positions per force evaluation -- not built here)", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = {"single": 200, "batched": 50, "serve": 0, "large": 2}[args.workload]
+        args.steps = {"single": 200, "batched": 50, "serve": 0, "large": 2,
+                      "copy": 1}[args.workload]
     gpu = nb.GpuContext(0)
     name, cus = gpu.device_info()
     log(f"device {name}, {cus} CUs")
     rec = {"single": run_single, "batched": run_batched, "serve": run_serve,
-           "large": run_large}[args.workload](args, gpu)
+           "large": run_large, "copy": run_copy}[args.workload](args, gpu)
     rec["device"] = {"arch": name, "compute_units": cus}
     print(json.dumps(rec), flush=True)
 
