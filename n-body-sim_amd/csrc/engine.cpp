@@ -49,6 +49,9 @@ Engine::~Engine() {
   drop_graphs();
   hipStreamSynchronize(stream_);
   hipFree(d_items_);
+  hipFree(d_step_items_);
+  hipFree(d_stage_);
+  hipHostFree(h_stage_);
   hipFree(d_diag_);
   hipFree(d_list_);
   hipFree(d_sys_);
@@ -197,6 +200,7 @@ static void validate(const SystemParams& p) {
 int Engine::add(const SystemParams& p, const double* m, const double* x,
                 const double* y, const double* z, const double* vx,
                 const double* vy, const double* vz) {
+  step_sig_.clear();
   validate(p);
   if (!m || !x || !y || !z || !vx || !vy || !vz)
     throw Error(NBODY_E_INVALID, "null body array");
@@ -260,6 +264,7 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
 }
 
 void Engine::remove(int id) {
+  step_sig_.clear();
   check_id(id);
   Sys& s = sys_[id];
   // Clear the range so stale values can never be read as sources.
@@ -292,6 +297,7 @@ void Engine::remove(int id) {
 }
 
 void Engine::set_params(int id, const SystemParams& p) {
+  step_sig_.clear();
   check_id(id);
   SystemParams q = p;
   q.n = sys_[id].p.n;
@@ -310,6 +316,7 @@ void Engine::set_params(int id, const SystemParams& p) {
 }
 
 void Engine::set_mode(int mode) {
+  step_sig_.clear();
   if (mode < MODE_F64_STRICT || mode > MODE_F32R)
     throw Error(NBODY_E_INVALID, "unknown precision");
   if (mode == mode_) return;
@@ -524,12 +531,31 @@ void Engine::prime(const std::vector<int>& ids) {
 }
 
 // Uploads the lists of step A (current parity) and step B (after one step)
-// into one device buffer.  Slot k = which*8 + round*2 + (0 force, 1 combine).
+// into the step-list device buffer.  Slot k = which*8 + round*2 + (0 force,
+// 1 combine).  The lists depend only on the listed systems' ids, sizes,
+// integrators, snapshot parity and the mode; when none of those changed
+// since the last upload the device copy is reused as it is.
 void Engine::upload_step_lists(const std::vector<int>& ids, StepLists& L) {
+  std::vector<int64_t> sig;
+  sig.reserve(4 * ids.size() + 2);
+  sig.push_back(mode_);
+  sig.push_back((int64_t)ids.size());
+  for (int id : ids) {
+    const Sys& s = sys_[id];
+    sig.push_back(id);
+    sig.push_back(s.off);
+    sig.push_back(s.p.n * 4 + s.p.integ);
+    sig.push_back(s.cur);
+  }
+  if (sig == step_sig_ && d_step_items_) {
+    L = step_lists_;
+    return;
+  }
   std::vector<WorkItem> fa[4], ca[4], fb[4], cb[4];
   step_lists(ids, 0, fa, ca);
   step_lists(ids, 1, fb, cb);
-  std::vector<WorkItem> all;
+  std::vector<WorkItem>& all = step_host_;
+  all.clear();
   for (int k = 0; k < 16; ++k) {
     const int which = k / 8, r = (k % 8) / 2, kind = k % 2;
     const std::vector<WorkItem>& l =
@@ -539,24 +565,37 @@ void Engine::upload_step_lists(const std::vector<int>& ids, StepLists& L) {
     all.insert(all.end(), l.begin(), l.end());
   }
   validate_items(all);
-  ensure_items(all.size());
+  // the previous lists may still be in use by queued launches
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  if (all.size() > step_items_cap_) {
+    hipFree(d_step_items_);
+    d_step_items_ = nullptr;
+    step_items_cap_ = std::max(all.size(), 2 * step_items_cap_);
+    hip_check(hipMalloc((void**)&d_step_items_,
+                        sizeof(WorkItem) * step_items_cap_),
+              "hipMalloc(step lists)");
+    drop_graphs();  // graphs hold the old buffer address
+  }
   if (!all.empty()) {
-    hip_check(hipMemcpyAsync(d_items_, all.data(), sizeof(WorkItem) * all.size(),
+    hip_check(hipMemcpyAsync(d_step_items_, all.data(),
+                             sizeof(WorkItem) * all.size(),
                              hipMemcpyHostToDevice, stream_),
-              "hipMemcpy(work lists)");
+              "hipMemcpy(step lists)");
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   }
+  step_sig_ = std::move(sig);
+  step_lists_ = L;
 }
 
 void Engine::launch_step(const StepLists& L, int which) {
   for (int r = 0; r < 4; ++r) {
     const int k = which * 8 + r * 2;
     if (L.cnt[k] == 0) continue;
-    hip_check(launch_force(mode_, d_arena_, d_items_ + L.off[k], L.cnt[k], d_sys_,
-                           stream_),
+    hip_check(launch_force(mode_, d_arena_, d_step_items_ + L.off[k], L.cnt[k],
+                           d_sys_, stream_),
               "force launch");
-    hip_check(launch_combine(mode_, d_arena_, d_items_ + L.off[k + 1], L.cnt[k + 1],
-                             d_sys_, stream_),
+    hip_check(launch_combine(mode_, d_arena_, d_step_items_ + L.off[k + 1],
+                             L.cnt[k + 1], d_sys_, stream_),
               "combine launch");
   }
 }
@@ -661,17 +700,54 @@ void Engine::download(int id, double* m, double* x, double* y, double* z,
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
-void Engine::download_positions_async(int id, double* x, double* y, double* z,
-                                      hipStream_t s) {
-  check_id(id);
-  const Sys& sy = sys_[id];
-  const size_t bytes = sizeof(double) * sy.p.n;
-  hip_check(hipMemcpyAsync(x, A_.x + sy.off, bytes, hipMemcpyDeviceToHost, s),
-            "hipMemcpy(snapshot)");
-  hip_check(hipMemcpyAsync(y, A_.y + sy.off, bytes, hipMemcpyDeviceToHost, s),
-            "hipMemcpy(snapshot)");
-  hip_check(hipMemcpyAsync(z, A_.z + sy.off, bytes, hipMemcpyDeviceToHost, s),
-            "hipMemcpy(snapshot)");
+const double* Engine::gather(const std::vector<int>& ids, int nfields,
+                             std::vector<int64_t>& rows, int64_t& total) {
+  if (nfields != 3 && nfields != 6)
+    throw Error(NBODY_E_INVALID, "gather: nfields must be 3 or 6");
+  rows.clear();
+  total = 0;
+  std::vector<WorkItem> items;
+  for (int id : ids) {
+    check_id(id);
+    rows.push_back(total);
+    const int64_t tiles = (sys_[id].p.n + kWave - 1) / kWave;
+    for (int64_t t = 0; t < tiles; ++t) {
+      WorkItem w{id, (int32_t)t, (int16_t)STAGE_ACC, (int8_t)0, (int8_t)0};
+      w.pad_ = (int32_t)total;
+      items.push_back(w);
+    }
+    total += sys_[id].p.n;
+  }
+  if (total == 0) return nullptr;
+  if (total > INT32_MAX) throw Error(NBODY_E_INVALID, "gather too large");
+  const size_t need = (size_t)total * nfields;
+  if (need > stage_cap_) {
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    hipFree(d_stage_);
+    hipHostFree(h_stage_);
+    d_stage_ = nullptr;
+    h_stage_ = nullptr;
+    stage_cap_ = std::max(need, 2 * stage_cap_);
+    hip_check(hipMalloc((void**)&d_stage_, sizeof(double) * stage_cap_),
+              "hipMalloc(gather staging)");
+    hip_check(hipHostMalloc((void**)&h_stage_, sizeof(double) * stage_cap_,
+                            hipHostMallocDefault),
+              "hipHostMalloc(gather staging)");
+  }
+  validate_items(items);
+  ensure_items(items.size());
+  hip_check(hipMemcpyAsync(d_items_, items.data(), sizeof(WorkItem) * items.size(),
+                           hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(work list)");
+  upload_table();
+  hip_check(launch_gather(d_arena_, d_items_, (int)items.size(), d_sys_,
+                          d_stage_, total, nfields, stream_),
+            "gather launch");
+  hip_check(hipMemcpyAsync(h_stage_, d_stage_, sizeof(double) * need,
+                           hipMemcpyDeviceToHost, stream_),
+            "hipMemcpy(gather)");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  return h_stage_;
 }
 
 void Engine::synchronize() {
@@ -691,7 +767,7 @@ void Engine::step_graph(const std::vector<int>& ids, int64_t nsteps) {
   prime(ids);
   StepLists L;
   upload_step_lists(ids, L);
-  int64_t key = (int64_t)(uintptr_t)d_items_ * 31 + mode_;
+  int64_t key = (int64_t)(uintptr_t)d_step_items_ * 31 + mode_;
   for (int k = 0; k < 16; ++k) key = key * 1000003 + L.cnt[k];
   if (key != gkey_ || ids != gids_ || !gexec_[0]) {
     drop_graphs();

@@ -74,9 +74,13 @@ class Engine {
   // Downloads the master state (any pointer may be null).  Synchronous.
   void download(int id, double* m, double* x, double* y, double* z,
                 double* vx, double* vy, double* vz);
-  // Asynchronous copy of positions into caller-owned (pinned) host memory.
-  void download_positions_async(int id, double* x, double* y, double* z,
-                                hipStream_t s);
+  // Gathers the state rows of several systems in one device pass and one
+  // device-to-host copy into library-owned pinned memory.  nfields 3:
+  // positions; 6: positions and velocities.  Returns a host view valid until
+  // the next gather: field f of system ids[k] starts at
+  // base[f * total + rows[k]].  Synchronous.
+  const double* gather(const std::vector<int>& ids, int nfields,
+                       std::vector<int64_t>& rows, int64_t& total);
   void synchronize();
 
   // Graph-captured multi-step loop for a fixed set of systems (batched API).
@@ -141,9 +145,19 @@ class Engine {
   size_t items_cap_ = 0;
   int32_t* d_list_ = nullptr;
   double* d_diag_ = nullptr;
+  double* d_stage_ = nullptr;   // gather staging (device)
+  double* h_stage_ = nullptr;   // gather staging (pinned host)
+  size_t stage_cap_ = 0;        // doubles
   std::vector<Sys> sys_;
   std::map<int64_t, int64_t> free_;  // offset -> length
   bool table_dirty_ = true;
+
+  // step lists: their own device buffer, reused while the signature holds
+  WorkItem* d_step_items_ = nullptr;
+  size_t step_items_cap_ = 0;
+  std::vector<WorkItem> step_host_;
+  std::vector<int64_t> step_sig_;
+  StepLists step_lists_;
 
   // graph cache for step_graph
   hipGraphExec_t gexec_[1] = {nullptr};

@@ -23,6 +23,11 @@ hipError_t launch_prepare(const Arena* A, const WorkItem* items, int nitems,
 // Per-system diagnostics (Diag layout), one workgroup per listed system.
 hipError_t launch_diag(const Arena* A, const SysDesc* sys, const int32_t* list,
                        int nlist, double* out, hipStream_t s);
+// Packs x, y, z (and vx, vy, vz when nfields == 6) of listed systems into
+// stage[field * total + row]; each item's pad_ holds its system's base row.
+hipError_t launch_gather(const Arena* A, const WorkItem* items, int nitems,
+                         const SysDesc* sys, double* stage, int64_t total,
+                         int nfields, hipStream_t s);
 hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s);
 hipError_t launch_probe(int* out, hipStream_t s);
 

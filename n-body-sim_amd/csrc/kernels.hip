@@ -711,6 +711,35 @@ __global__ __launch_bounds__(kBlock) void k_diag(const Arena* Ap, const SysDesc*
   }
 }
 
+// Packs state rows of listed systems into a staging buffer: item.pad_ is
+// the system's base row in the staging layout [field][total]; fields 0..2
+// are x, y, z and 3..5 (when nfields == 6) vx, vy, vz.
+__global__ __launch_bounds__(kBlock) void k_gather(const Arena* Ap,
+                                                   const WorkItem* items,
+                                                   int nitems,
+                                                   const SysDesc* sysd,
+                                                   double* stage,
+                                                   int64_t total,
+                                                   int nfields) {
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int idx = (int)blockIdx.x * kWavesPerBlock + wave;
+  if (idx >= nitems) return;
+  const WorkItem it = load_uniform(items + idx);
+  const CArena* A = (const CArena*)Ap;
+  const CSys* sd = (const CSys*)(sysd + it.sys);
+  const int64_t k = (int64_t)it.tile * kWave + (threadIdx.x & (kWave - 1));
+  if (k >= sd->n) return;
+  const int64_t i = sd->off + k, r = (int64_t)it.pad_ + k;
+  stage[r] = A->x[i];
+  stage[total + r] = A->y[i];
+  stage[2 * total + r] = A->z[i];
+  if (nfields == 6) {
+    stage[3 * total + r] = A->vx[i];
+    stage[4 * total + r] = A->vy[i];
+    stage[5 * total + r] = A->vz[i];
+  }
+}
+
 // Writes zeros; used instead of hipMemsetAsync so it can sit in a graph.
 __global__ void k_fill_zero(double* p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -804,6 +833,15 @@ hipError_t launch_diag(const Arena* A, const SysDesc* sys, const int32_t* list,
   if (nlist <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_diag, dim3(nlist), dim3(kBlock), 0, s, A, sys, list,
                      out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const Arena* A, const WorkItem* items, int nitems,
+                         const SysDesc* sys, double* stage, int64_t total,
+                         int nfields, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather, dim3(blocks_for(nitems)), dim3(kBlock), 0, s, A,
+                     items, nitems, sys, stage, total, nfields);
   return hipGetLastError();
 }
 

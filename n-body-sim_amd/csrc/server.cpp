@@ -272,6 +272,7 @@ class Server {
 
   void run() {
     hipSetDevice(cfg_.device);
+    std::vector<int> prev_ids;
     for (;;) {
       std::vector<Request> admit;
       {
@@ -287,7 +288,7 @@ class Server {
         if (stop_) return;
         admit = take_admissible();
       }
-      // upload admitted requests (engine is owned by this thread)
+      // upload admitted requests (the engine is owned by this thread)
       for (auto& q : admit) {
         Active a;
         a.t_admit = now_s();
@@ -296,108 +297,175 @@ class Server {
                             q.z.data(), q.vx.data(), q.vy.data(), q.vz.data());
           eng_->set_rebase_interval(a.sys, cfg_.snapshot_every);
         } catch (const std::exception& ex) {
-          std::lock_guard<std::mutex> lk(mu_);
           Event e;
           fill(e.ev, NBODY_EVENT_FAILED, q, 0, a.t_admit, ex.what());
+          std::lock_guard<std::mutex> lk(mu_);
           push_event(std::move(e));
           ++failed_;
           continue;
         }
-        // free the host copy of the initial state
         for (auto* v : {&q.m, &q.x, &q.y, &q.z, &q.vx, &q.vy, &q.vz})
-          std::vector<double>().swap(*v);
+          std::vector<double>().swap(*v);  // free the host copy
         a.req = std::move(q);
         std::lock_guard<std::mutex> lk(mu_);
         ++admitted_;
         const uint64_t id = a.req.id;
         active_.emplace(id, std::move(a));
       }
-      // retire cancelled and zero-step requests, choose the epoch length
+      // retire cancelled and finished requests, then choose the epoch length
+      retire_finished();
       std::vector<int> ids;
       int64_t k = cfg_.snapshot_every;
       {
         std::lock_guard<std::mutex> lk(mu_);
-        for (auto it = active_.begin(); it != active_.end();) {
-          Active& a = it->second;
+        for (auto& kv : active_) {
+          const Active& a = kv.second;
           const int64_t done = eng_->steps_done(a.sys);
-          if (a.cancel || done >= a.req.steps) {
-            retire(a, a.cancel ? NBODY_EVENT_CANCELLED : NBODY_EVENT_DONE);
-            it = active_.erase(it);
-            continue;
-          }
           ids.push_back(a.sys);
           k = std::min(k, cfg_.snapshot_every - done % cfg_.snapshot_every);
           k = std::min(k, a.req.steps - done);
-          ++it;
         }
       }
       if (ids.empty()) continue;
       std::sort(ids.begin(), ids.end());
-      double inter = 0;
       try {
-        eng_->step_graph(ids, k);
+        // a graph pays off only while the resident set stays the same
+        if (ids == prev_ids)
+          eng_->step_graph(ids, k);
+        else
+          eng_->step(ids, k);
         eng_->synchronize();
       } catch (const std::exception& ex) {
         std::lock_guard<std::mutex> lk(mu_);
         for (auto it = active_.begin(); it != active_.end();) {
-          retire(it->second, NBODY_EVENT_FAILED, ex.what());
+          Event e;
+          fill(e.ev, NBODY_EVENT_FAILED, it->second.req,
+               eng_->steps_done(it->second.sys), it->second.t_admit, ex.what());
+          try {
+            eng_->remove(it->second.sys);
+          } catch (...) {
+          }
+          push_event(std::move(e));
           ++failed_;
           it = active_.erase(it);
         }
+        prev_ids.clear();
         continue;
       }
-      // snapshots at own-step multiples of snapshot_every, and completions
-      std::lock_guard<std::mutex> lk(mu_);
-      epoch_steps_ += k;
-      for (auto it = active_.begin(); it != active_.end();) {
-        Active& a = it->second;
-        const double n = (double)a.req.p.n;
-        inter += n * (n - 1.0) * (double)(k * evals_per_step(a.req.p.integ));
-        const int64_t done = eng_->steps_done(a.sys);
-        if (done >= a.req.steps) {
-          retire(a, NBODY_EVENT_DONE);
-          it = active_.erase(it);
-          continue;
+      prev_ids = ids;
+      // snapshots at own-step multiples of snapshot_every (completions are
+      // retired at the top of the next iteration)
+      std::vector<int> snap_sys;
+      std::vector<const Active*> snap_req;
+      double inter = 0;
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        epoch_steps_ += k;
+        for (auto& kv : active_) {
+          const Active& a = kv.second;
+          const double n = (double)a.req.p.n;
+          inter += n * (n - 1.0) * (double)(k * evals_per_step(a.req.p.integ));
+          const int64_t done = eng_->steps_done(a.sys);
+          if (a.req.want_snapshots && done < a.req.steps &&
+              done % cfg_.snapshot_every == 0) {
+            snap_sys.push_back(a.sys);
+            snap_req.push_back(&a);
+          }
         }
-        if (a.req.want_snapshots && done % cfg_.snapshot_every == 0) {
-          Event e;
-          fill(e.ev, NBODY_EVENT_SNAPSHOT, a.req, done, a.t_admit, "");
-          e.x.resize(a.req.p.n);
-          e.y.resize(a.req.p.n);
-          e.z.resize(a.req.p.n);
-          eng_->download(a.sys, nullptr, e.x.data(), e.y.data(), e.z.data(),
-                         nullptr, nullptr, nullptr);
-          push_event(std::move(e));
-        }
-        ++it;
+        interactions_ += inter;
       }
-      interactions_ += inter;
+      if (!snap_sys.empty()) {
+        std::vector<int64_t> rows;
+        int64_t total = 0;
+        std::vector<Event> evs;
+        try {
+          const double* h = eng_->gather(snap_sys, 3, rows, total);
+          for (size_t q = 0; q < snap_sys.size(); ++q) {
+            const Active& a = *snap_req[q];
+            const int64_t n = a.req.p.n;
+            Event e;
+            fill(e.ev, NBODY_EVENT_SNAPSHOT, a.req, eng_->steps_done(a.sys),
+                 a.t_admit, "");
+            e.x.assign(h + rows[q], h + rows[q] + n);
+            e.y.assign(h + total + rows[q], h + total + rows[q] + n);
+            e.z.assign(h + 2 * total + rows[q], h + 2 * total + rows[q] + n);
+            evs.push_back(std::move(e));
+          }
+        } catch (const std::exception&) {
+          evs.clear();  // snapshots are best effort; the final state is not
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto& e : evs) push_event(std::move(e));
+      }
     }
   }
 
-  // Emits the final event of an active request and frees its slot (mu_ held).
-  void retire(Active& a, int kind, const char* msg = "") {
-    Event e;
-    const int64_t done = eng_->steps_done(a.sys);
-    fill(e.ev, kind, a.req, done, a.t_admit, msg);
-    if (kind == NBODY_EVENT_DONE) {
-      const int64_t n = a.req.p.n;
-      for (auto* v : {&e.x, &e.y, &e.z, &e.vx, &e.vy, &e.vz}) v->resize(n);
-      try {
-        eng_->download(a.sys, nullptr, e.x.data(), e.y.data(), e.z.data(),
-                       e.vx.data(), e.vy.data(), e.vz.data());
-        ++completed_;
-      } catch (const std::exception& ex) {
-        fill(e.ev, NBODY_EVENT_FAILED, a.req, done, a.t_admit, ex.what());
-        for (auto* v : {&e.x, &e.y, &e.z, &e.vx, &e.vy, &e.vz}) v->clear();
-        ++failed_;
+  // Emits the final events of cancelled and finished requests (final state
+  // gathered for all of them in one pass) and frees their arena ranges.
+  void retire_finished() {
+    std::vector<uint64_t> fin, canc;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (auto& kv : active_) {
+        const Active& a = kv.second;
+        if (a.cancel)
+          canc.push_back(kv.first);
+        else if (eng_->steps_done(a.sys) >= a.req.steps)
+          fin.push_back(kv.first);
       }
     }
-    try {
-      eng_->remove(a.sys);
-    } catch (...) {
+    if (fin.empty() && canc.empty()) return;
+    std::vector<Event> evs;
+    std::vector<int> sys;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (uint64_t id : fin) sys.push_back(active_.at(id).sys);
     }
-    push_event(std::move(e));
+    std::vector<int64_t> rows;
+    int64_t total = 0;
+    const double* h = nullptr;
+    std::string err;
+    if (!sys.empty()) {
+      try {
+        h = eng_->gather(sys, 6, rows, total);
+      } catch (const std::exception& ex) {
+        err = ex.what();
+      }
+    }
+    std::lock_guard<std::mutex> lk(mu_);
+    for (size_t q = 0; q < fin.size(); ++q) {
+      Active& a = active_.at(fin[q]);
+      Event e;
+      const int64_t done = eng_->steps_done(a.sys);
+      if (h) {
+        fill(e.ev, NBODY_EVENT_DONE, a.req, done, a.t_admit, "");
+        const int64_t n = a.req.p.n;
+        std::vector<double>* f[6] = {&e.x, &e.y, &e.z, &e.vx, &e.vy, &e.vz};
+        for (int c = 0; c < 6; ++c)
+          f[c]->assign(h + c * total + rows[q], h + c * total + rows[q] + n);
+        ++completed_;
+      } else {
+        fill(e.ev, NBODY_EVENT_FAILED, a.req, done, a.t_admit, err.c_str());
+        ++failed_;
+      }
+      evs.push_back(std::move(e));
+    }
+    for (uint64_t id : canc) {
+      Active& a = active_.at(id);
+      Event e;
+      fill(e.ev, NBODY_EVENT_CANCELLED, a.req, eng_->steps_done(a.sys),
+           a.t_admit, "cancelled");
+      evs.push_back(std::move(e));
+    }
+    for (auto* lst : {&fin, &canc})
+      for (uint64_t id : *lst) {
+        try {
+          eng_->remove(active_.at(id).sys);
+        } catch (...) {
+        }
+        active_.erase(id);
+      }
+    for (auto& e : evs) push_event(std::move(e));
   }
 
   nbody_server_config cfg_;
