@@ -83,6 +83,16 @@ int nbody_ctx_device_info(nbody_ctx *ctx, char *name, int name_len,
 int nbody_ctx_record(nbody_ctx *ctx, int slot);
 int nbody_ctx_elapsed(nbody_ctx *ctx, int slot_from, int slot_to, float *ms);
 
+/* ---- tuning (no reference counterpart): source-range parts per target
+ * tile for systems of n bodies, by bucket b = floor(log2 n), b = 0..31.
+ * Loaded once per process from tuning/gfx950.txt next to the library (or the
+ * file named by NBODY_AMD_TUNING), else a measured default rule.  The parts
+ * set the summation order of NBODY_PRECISION_F64 / F32C / F32R (never of
+ * F64_STRICT), so a fixed table gives fixed bits; a change applies to
+ * objects created afterwards. */
+int nbody_tuning_set_split(const int *parts, int count);
+int nbody_tuning_get_split(int *parts, int count);
+
 /* ---- device bodies: replaces DeviceBodies (src/gpu/device_bodies.rs:7-55).
  * create = DeviceBodies::new (:23-36, upload of the 7 SoA arrays plus zeroed
  * accelerations); download = DeviceBodies::download (:40-50) plus the

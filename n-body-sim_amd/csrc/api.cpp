@@ -13,6 +13,7 @@
 #include "engine.h"
 #include "kernels.h"
 #include "server.h"
+#include "tuning.h"
 
 using nbody::Engine;
 using nbody::Error;
@@ -210,6 +211,21 @@ int nbody_ctx_elapsed(nbody_ctx* ctx, int a, int b, float* ms) {
     hip_check(hipEventSynchronize(ctx->ev[b]), "hipEventSynchronize");
     hip_check(hipEventElapsedTime(ms, ctx->ev[a], ctx->ev[b]),
               "hipEventElapsedTime");
+  });
+}
+
+// ---- tuning ---------------------------------------------------------------
+int nbody_tuning_set_split(const int* parts, int count) {
+  return guarded([&] {
+    require(parts && count > 0, NBODY_E_INVALID, "bad tuning table");
+    nbody::tuning_set(parts, count);
+  });
+}
+
+int nbody_tuning_get_split(int* parts, int count) {
+  return guarded([&] {
+    require(parts && count > 0, NBODY_E_INVALID, "bad tuning table");
+    nbody::tuning_get(parts, count);
   });
 }
 

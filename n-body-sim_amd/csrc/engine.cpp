@@ -9,6 +9,7 @@
 
 #include "../../include/nbody.h"
 #include "kernels.h"
+#include "tuning.h"
 
 namespace nbody {
 
@@ -23,6 +24,8 @@ constexpr int64_t kAlign = 64;  // system ranges start on a wave boundary
 
 int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 }  // namespace
+
+static int split_for(int64_t n);
 
 Engine::Engine(int device, hipStream_t stream, int mode, int64_t capacity,
                int max_systems)
@@ -231,6 +234,7 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
   s.p = p;
   s.off = off;
   s.span = need;
+  s.split = split_for(p.n);
   // centre of mass and its velocity, f64 in index order (origin of the f32
   // snapshots; advanced by vcom * elapsed time at every re-centring)
   double M = 0, c[3] = {0, 0, 0}, v[3] = {0, 0, 0};
@@ -350,19 +354,22 @@ SysDesc Engine::desc_of(int id) const {
   return d;
 }
 
-// Source-range parts per tile for one system.  Depends only on the system's
-// own size and the arithmetic mode (never on what shares the launch), so a
+// Source-range parts per tile for one system: looked up by the bucket
+// floor(log2 n) in the process-wide split table (tuning.cpp), then clamped so
+// every part keeps at least 64 sources.  Depends only on the system's own
+// size and the arithmetic mode (never on what shares the launch), so a
 // system's summation order -- and its bits -- are the same alone or in a
-// batch.  Aim for about kTargetWaves waves per system so a single system
-// fills the GPU (4 waves per SIMD on 256 CUs), with at least kMinPart sources
-// per part; strict mode keeps the reference's single sequential sum.
+// batch.  Strict mode keeps the reference's single sequential sum.
 int Engine::split_of(int id) const {
-  if (mode_ == MODE_F64_STRICT) return 1;
-  constexpr int64_t kTargetWaves = 4096, kMinPart = 512;
-  const int64_t n = sys_[id].p.n;
-  const int64_t tiles = (n + kWave - 1) / kWave;
-  int64_t k = std::min(kTargetWaves / tiles, n / kMinPart);
-  return (int)std::max<int64_t>(1, std::min<int64_t>(k, kMaxSplit));
+  return mode_ == MODE_F64_STRICT ? 1 : sys_[id].split;
+}
+
+// The split a system gets for its lifetime, fixed when it is uploaded so the
+// host work lists and the device descriptor can never disagree.
+static int split_for(int64_t n) {
+  int k = tuning_split(n);
+  while (k > 1 && n / k < 64) --k;
+  return std::max(1, std::min(k, kMaxSplit));
 }
 
 void Engine::upload_table() {

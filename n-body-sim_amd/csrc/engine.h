@@ -99,6 +99,7 @@ class Engine {
     bool prepared = false;
     int64_t steps = 0;
     int64_t rebase_every = 1;
+    int split = 1;              // source-range parts (fixed at upload)
     double com[3] = {0, 0, 0};  // centre of mass at upload
     double vcom[3] = {0, 0, 0};  // centre-of-mass velocity
   };

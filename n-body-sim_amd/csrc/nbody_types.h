@@ -20,7 +20,12 @@
 namespace nbody {
 
 constexpr int kWave = 64;        // gfx950 wavefront
-constexpr int kBlock = 256;      // 4 waves per workgroup, one work item each
+#ifndef NBODY_BLOCK
+#define NBODY_BLOCK 64
+#endif
+// threads per workgroup: one wave per workgroup measured best (finer-grained
+// dispatch of uneven work items); each wave runs one work item
+constexpr int kBlock = NBODY_BLOCK;
 constexpr int kPad = 128;        // slack elements after the arena
 constexpr int kMaxSplit = 8;     // source-range parts per target tile
 

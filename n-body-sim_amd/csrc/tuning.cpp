@@ -1,0 +1,103 @@
+// Tuning table: source-range parts per target tile, by size bucket.
+//
+// The default reproduces the rule measured on MI355X: aim for ~16384 waves per
+// system with at least 256 sources per part, at most kMaxSplit parts.  An
+// autotuned table (n-body-sim_amd/tuning/gfx950.txt, written by
+// nbody_amd/autotune.py from timings on the GPU) replaces it when present next
+// to the library or named by NBODY_AMD_TUNING.  The table is read once per
+// process; results depend on it (summation order), so a given table always
+// gives the same bits.
+#include "tuning.h"
+
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "nbody_types.h"
+
+namespace nbody {
+namespace {
+
+std::once_flag g_once;
+int g_split[kTuningBuckets];
+
+int default_split(int b) {
+  const int64_t n = (int64_t)1 << b;
+  const int64_t tiles = (n + kWave - 1) / kWave;
+  int64_t k = std::min<int64_t>(16384 / std::max<int64_t>(tiles, 1), n / 256);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(k, kMaxSplit));
+}
+
+std::string library_dir() {
+  Dl_info info;
+  if (dladdr((void*)&default_split, &info) && info.dli_fname) {
+    std::string p = info.dli_fname;
+    const size_t s = p.find_last_of('/');
+    return s == std::string::npos ? std::string(".") : p.substr(0, s);
+  }
+  return ".";
+}
+
+void init() {
+  for (int b = 0; b < kTuningBuckets; ++b) g_split[b] = default_split(b);
+  const char* env = std::getenv("NBODY_AMD_TUNING");
+  if (env && *env) {
+    tuning_load(env);
+  } else {
+    tuning_load((library_dir() + "/tuning/gfx950.txt").c_str());
+  }
+}
+
+int bucket_of(int64_t n) {
+  int b = 0;
+  while (b + 1 < kTuningBuckets && ((int64_t)1 << (b + 1)) <= n) ++b;
+  return b;
+}
+
+}  // namespace
+
+int tuning_split(int64_t n) {
+  std::call_once(g_once, init);
+  return g_split[bucket_of(std::max<int64_t>(n, 1))];
+}
+
+void tuning_set(const int* parts, int count) {
+  std::call_once(g_once, init);
+  for (int b = 0; b < kTuningBuckets && b < count; ++b)
+    g_split[b] = std::max(1, std::min(parts[b], kMaxSplit));
+}
+
+void tuning_get(int* parts, int count) {
+  std::call_once(g_once, init);
+  for (int b = 0; b < kTuningBuckets && b < count; ++b) parts[b] = g_split[b];
+}
+
+// File format: lines "split <bucket> <parts>"; '#' starts a comment.
+bool tuning_load(const char* path) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) return false;
+  char line[256];
+  int loaded = 0;
+  int tmp[kTuningBuckets];
+  std::memcpy(tmp, g_split, sizeof(tmp));
+  while (std::fgets(line, sizeof(line), f)) {
+    if (line[0] == '#') continue;
+    int b = -1, k = 0;
+    if (std::sscanf(line, "split %d %d", &b, &k) == 2 && b >= 0 &&
+        b < kTuningBuckets && k >= 1) {
+      tmp[b] = std::min(k, kMaxSplit);
+      ++loaded;
+    }
+  }
+  std::fclose(f);
+  if (loaded == 0) return false;
+  std::memcpy(g_split, tmp, sizeof(tmp));
+  return true;
+}
+
+}  // namespace nbody

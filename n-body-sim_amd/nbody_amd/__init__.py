@@ -61,6 +61,8 @@ def _bind(lib):
         "nbody_ctx_device_info": ([vp, ctypes.c_char_p, i, ctypes.POINTER(i)], i),
         "nbody_ctx_record": ([vp, i], i),
         "nbody_ctx_elapsed": ([vp, i, i, ctypes.POINTER(ctypes.c_float)], i),
+        "nbody_tuning_set_split": ([ctypes.POINTER(i), i], i),
+        "nbody_tuning_get_split": ([ctypes.POINTER(i), i], i),
         "nbody_bodies_create": ([vp, i64] + [_D] * 7 + [pp], i),
         "nbody_bodies_download": ([vp, vp] + [_D] * 7, i),
         "nbody_bodies_destroy": ([vp], i),
@@ -391,6 +393,23 @@ class GpuSimulation:
             self.gpu.handle, self.bodies.handle, self.gravity.g_constant,
             self.gravity.eps2, _p(out)))
         return out
+
+
+# ---- tuning -----------------------------------------------------------------
+TUNING_BUCKETS = 32
+
+
+def tuning_split() -> list:
+    """Source-range parts per tile by bucket floor(log2 n) (process-wide)."""
+    arr = (ctypes.c_int * TUNING_BUCKETS)()
+    _check(load_library().nbody_tuning_get_split(arr, TUNING_BUCKETS))
+    return list(arr)
+
+
+def set_tuning_split(parts) -> None:
+    """Replace the table; applies to systems uploaded afterwards."""
+    arr = (ctypes.c_int * TUNING_BUCKETS)(*parts)
+    _check(load_library().nbody_tuning_set_split(arr, TUNING_BUCKETS))
 
 
 # ---- scenarios ------------------------------------------------------------

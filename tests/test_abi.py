@@ -160,3 +160,29 @@ def test_bodies_length_check():
     with pytest.raises(ValueError):
         nb.Bodies(np.ones(3), np.ones(3), np.ones(2), np.ones(3), np.ones(3),
                   np.ones(3), np.ones(3))
+
+
+def test_tuning_table_file_and_abi(tmp_path):
+    """The split table loads from NBODY_AMD_TUNING (format `split b k`), is
+    clamped to 1..8 parts, and set/get round-trip through the C ABI."""
+    import subprocess
+    import sys
+    f = tmp_path / "t.txt"
+    f.write_text("# test table\nsplit 10 3\nsplit 12 99\nsplit 5 1\n")
+    code = ("import sys; sys.path.insert(0, %r); import nbody_amd as nb; "
+            "t = nb.tuning_split(); print(t[10], t[12], t[5]); "
+            "t[7] = 2; nb.set_tuning_split(t); print(nb.tuning_split()[7])"
+            % nb._PKG_DIR)
+    env = dict(__import__("os").environ, NBODY_AMD_TUNING=str(f))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
+                         text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split() == ["3", "8", "1", "2"]
+
+
+def test_committed_tuning_table_parses():
+    import os
+    path = os.path.join(nb._PKG_DIR, "tuning", "gfx950.txt")
+    rows = [ln.split() for ln in open(path) if ln.startswith("split ")]
+    assert len(rows) == 32
+    assert all(1 <= int(k) <= 8 for _, _, k in rows)
