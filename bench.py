@@ -77,8 +77,9 @@ def time_single(gpu, b, prec, g, eps, dt, steps, warmup):
     gpu.record(1)
     ms = gpu.elapsed_ms(0, 1)
     e1 = energy(gpu, db, g, eps)
+    final = db.download()
     db.close()
-    return ms, acc0, e0, e1
+    return ms, acc0, e0, e1, final
 
 
 def energy(gpu, db, g, eps):
@@ -120,12 +121,15 @@ def run_single(args, gpu):
     res = {}
     for prec in (nb.Precision.F32C, nb.Precision.F64):
         log(f"timing {prec.name}: N={n} VV {args.steps} steps (+{args.warmup} warmup)")
-        ms, acc0, e0, e1 = time_single(gpu, b, prec, g, eps, dt, args.steps, args.warmup)
-        res[prec] = (ms, acc0, e0, e1)
+        ms, acc0, e0, e1, fin = time_single(gpu, b, prec, g, eps, dt, args.steps,
+                                            args.warmup)
+        res[prec] = (ms, acc0, e0, e1, fin)
         log(f"  {ms:.1f} ms -> {inter / (ms / 1e3):.4e} int/s")
     traffic = measured_traffic()
-    ms32, acc32, e0_32, e1_32 = res[nb.Precision.F32C]
-    ms64, acc64, e0_64, e1_64 = res[nb.Precision.F64]
+    ms32, acc32, e0_32, e1_32, fin32 = res[nb.Precision.F32C]
+    ms64, acc64, e0_64, e1_64, fin64 = res[nb.Precision.F64]
+    dev = np.sqrt((fin32.pos_x - fin64.pos_x) ** 2 + (fin32.pos_y - fin64.pos_y) ** 2 +
+                  (fin32.pos_z - fin64.pos_z) ** 2)
     rate32 = inter / (ms32 / 1e3)
     rate64 = inter / (ms64 / 1e3)
     cpu = None
@@ -142,6 +146,11 @@ def run_single(args, gpu):
     drift = lambda e0, e1: abs(e1[2] - e0[2]) / abs(e0[2])  # noqa: E731
     acc_rep["f32c_energy_drift"] = drift(e0_32, e1_32)
     acc_rep["f64_energy_drift"] = drift(e0_64, e1_64)
+    # trajectories after warmup + steps: f32c against f64, in units of the
+    # Plummer scale radius (1)
+    acc_rep["trajectory_max_dev_f32c_vs_f64"] = float(dev.max())
+    acc_rep["trajectory_rms_dev_f32c_vs_f64"] = float(np.sqrt(np.mean(dev ** 2)))
+    acc_rep["trajectory_steps"] = args.warmup + args.steps
     return {
         "metric": "pair-interactions/s (f32-compensated forces, f64 state)",
         "value": rate32, "unit": "pair-interactions/s", "n_gpus": 1,
@@ -284,6 +293,22 @@ def run_large(args, gpu):
     n, g, eps, dt = 1 << 20, 1.0, 0.002, 2.5e-4
     b = nb.colliding(n, 5, separation=10.0, rel_speed=0.5)
     out = {}
+    # sampled-target oracle: 4096 random targets against all sources (f64 CPU)
+    acc_rep = {}
+    if not args.no_cpu_baseline:
+        import oracle
+        idx = np.sort(np.random.default_rng(9).choice(n, 4096, replace=False))
+        t0 = time.perf_counter()
+        ref = oracle.newton_targets(b.masses, b.pos_x, b.pos_y, b.pos_z, g, eps, idx,
+                                    min(16, oracle.default_threads()))
+        log(f"sampled oracle: 4096 targets in {time.perf_counter() - t0:.1f} s")
+        for prec in (nb.Precision.F32C, nb.Precision.F64):
+            db = nb.DeviceBodies(gpu, b)
+            got = nb.GpuNewtonParallelGravity(g, eps * eps, prec).calculate_accelerations(gpu, db)
+            db.close()
+            mx, med = rel_err(tuple(a[idx] for a in got), ref)
+            acc_rep[f"{prec.name.lower()}_max_rel_sampled"] = mx
+            acc_rep[f"{prec.name.lower()}_median_rel_sampled"] = med
     for prec in (nb.Precision.F32C, nb.Precision.F64):
         grav = nb.GpuNewtonParallelGravity(g, eps * eps, prec)
         integ = nb.GpuRungeKuttaIntegrator(dt)
@@ -317,7 +342,7 @@ def run_large(args, gpu):
         "cpu_baseline": None,
         "fp64_plain": {"value": rate64, "f32c_speedup": rate32 / rate64,
                        "roofline": roofline(rate64, "f64", ms64 / args.steps / 4)},
-        "accuracy": {"f32c_energy_drift": drift32, "f64_energy_drift": drift64},
+        "accuracy": dict(acc_rep, f32c_energy_drift=drift32, f64_energy_drift=drift64),
     }
 
 
