@@ -31,7 +31,7 @@ extern "C" {
 
 /* ---- status codes ---------------------------------------------------- */
 #define NBODY_OK 0
-#define NBODY_E_INVALID (-1)      /* bad argument (N=0, non-finite value, dt<=0, ...) */
+#define NBODY_E_INVALID (-1)      /* bad argument (N=0, non-finite value or dt, ...) */
 #define NBODY_E_UNSUPPORTED (-2)  /* method/integrator combination not offered on GPU */
 #define NBODY_E_DEVICE (-3)       /* HIP runtime error / no usable GPU */
 #define NBODY_E_NOMEM (-4)        /* device allocation failed */
@@ -130,6 +130,20 @@ int nbody_calculate_accelerations(nbody_ctx *ctx, nbody_bodies *b,
 int nbody_step(nbody_ctx *ctx, nbody_bodies *b, int integrator,
                int gravity_method, int precision, double dt, double g,
                double eps2, int64_t nsteps);
+
+/* ---- stepping with a snapshot after every step: replaces the loop of
+ * SimulationRunner::run (src/simulation_runner.rs:28-52) over
+ * GpuSimulation::step and ::snapshot (src/simulation/gpu_simulation.rs:45-65),
+ * which copies all seven state arrays to the host after every step.  Here
+ * positions of step s (1-based step s+1) land at pos_x[s*n .. s*n+n) and
+ * likewise pos_y/pos_z, each an array of nsteps*n doubles; the steps run back
+ * to back and the positions leave the device in one copy per field for every
+ * `max_stage_bytes` of device staging (0: 64 MiB; at least one step).
+ * Synchronous.  Same arguments and errors as nbody_step. */
+int nbody_step_record(nbody_ctx *ctx, nbody_bodies *b, int integrator,
+                      int gravity_method, int precision, double dt, double g,
+                      double eps2, int64_t nsteps, double *pos_x,
+                      double *pos_y, double *pos_z, int64_t max_stage_bytes);
 
 /* ---- diagnostics (no reference counterpart): f64 on the GPU.
  * out[0] kinetic K, out[1] softened potential U, out[2] E = K + U,
@@ -237,6 +251,45 @@ int nbody_server_cancel(nbody_server *s, uint64_t request_id);
  * [7] failed. */
 int nbody_server_stats(nbody_server *s, double out[8]);
 int nbody_server_destroy(nbody_server *s);
+
+/* ---- snapshot files: replaces SimulationDataWriter
+ * (src/output.rs:73-139).  No GPU involved.  The format follows the path's
+ * extension, compared case-insensitively: ".csv" writes the header
+ * "time,id,x,y,z" with the first record, then one "\n"-ended line per body
+ * per snapshot, floats in shortest round-trip form in the layout of the
+ * reference's csv writer (src/output.rs:106-122); ".nbody" writes the magic
+ * "NBODY001" and 40-byte little-endian records f64 time, u64 id, f64 x,
+ * f64 y, f64 z (src/output.rs:124-138).  Other extensions return
+ * NBODY_E_INVALID (the reference's "Unsupported output file format"). */
+typedef struct nbody_writer nbody_writer;
+int nbody_writer_open(const char *path, nbody_writer **out);
+/* One snapshot: records (time, id, x[id], y[id], z[id]) for id 0..n-1,
+ * the SimulationData::extend_from_snapshot order (src/output.rs:46-54). */
+int nbody_writer_append(nbody_writer *w, double time, int64_t n,
+                        const double *pos_x, const double *pos_y,
+                        const double *pos_z);
+/* Flushes, closes and frees (also on error). */
+int nbody_writer_close(nbody_writer *w);
+/* Text of one float as the snapshot CSV holds it (buf: 32 bytes, NUL-ended). */
+int nbody_format_f64(double v, char buf[32]);
+/* Parses a NUL-ended field with Rust's f64::from_str grammar, which the
+ * reference applies to CSV fields and numeric flags: optional sign, then
+ * inf / infinity / nan (any case) or decimal digits with an optional '.' and
+ * exponent; nothing else, no spaces.  NBODY_E_INVALID when it is not one. */
+int nbody_parse_f64(const char *text, double *out);
+
+/* ---- initial conditions: replaces input::load_bodies (src/input.rs:20-33).
+ * CSV with a header naming mass,pos_x,pos_y,pos_z,vel_x,vel_y,vel_z in any
+ * order (other columns ignored).  With any array null, only *n is set to the
+ * number of bodies.  Otherwise *n is the arrays' capacity on entry and the
+ * count on return (NBODY_E_NOSPACE, with *n the count, when they are too
+ * small).  NBODY_E_NOTFOUND when the file cannot be opened; NBODY_E_INVALID
+ * with the line in nbody_last_error() for a malformed file. */
+int nbody_load_initial_conditions(const char *path, int64_t *n,
+                                  double *masses, double *pos_x,
+                                  double *pos_y, double *pos_z,
+                                  double *vel_x, double *vel_y,
+                                  double *vel_z);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,7 @@
 
 #include "../../include/nbody.h"
 #include "engine.h"
+#include "formats.h"
 #include "kernels.h"
 #include "server.h"
 #include "tuning.h"
@@ -51,7 +52,16 @@ struct nbody_ctx {
   std::string name;
   int cus = 0;
   hipEvent_t ev[8] = {};
-  void use() const { hip_check(hipSetDevice(device), "hipSetDevice"); }
+  // Bodies and batches made from this context share its stream.  Destroying
+  // the context while some live only marks it closed; the last of them to be
+  // destroyed frees it, so destroy order does not matter to callers.
+  int children = 0;
+  bool closed = false;
+  void use() const {
+    if (closed) throw Error(NBODY_E_STATE, "context was destroyed");
+    set_device();
+  }
+  void set_device() const { hip_check(hipSetDevice(device), "hipSetDevice"); }
 };
 
 struct nbody_bodies {
@@ -59,6 +69,10 @@ struct nbody_bodies {
   std::unique_ptr<Engine> eng;
   int id = -1;
   int64_t n = 0;
+};
+
+struct nbody_writer {
+  std::unique_ptr<nbody::SnapshotWriter> w;
 };
 
 struct nbody_batch {
@@ -108,9 +122,10 @@ void check_gravity(int gravity_method) {
   throw Error(NBODY_E_INVALID, "unknown gravity method");
 }
 
+// Any finite step is accepted, as by the reference's integrators (a
+// negative step runs the system backwards, zero leaves it in place).
 void check_dt(double dt) {
-  require(std::isfinite(dt) && dt > 0.0, NBODY_E_INVALID,
-          "time step must be finite and > 0");
+  require(std::isfinite(dt), NBODY_E_INVALID, "time step must be finite");
 }
 }  // namespace
 
@@ -160,15 +175,37 @@ int nbody_ctx_create(int device, nbody_ctx** out) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+void free_ctx(nbody_ctx* ctx) {
+  ctx->set_device();
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+// Called after a child object is gone.
+void release_child(nbody_ctx* ctx) {
+  if (--ctx->children == 0 && ctx->closed) free_ctx(ctx);
+}
+}  // namespace
+
+extern "C" {
+
 int nbody_ctx_destroy(nbody_ctx* ctx) {
   return guarded([&] {
     if (!ctx) return;
-    ctx->use();
-    hipStreamSynchronize(ctx->stream);
-    for (auto& e : ctx->ev)
-      if (e) hipEventDestroy(e);
-    hipStreamDestroy(ctx->stream);
-    delete ctx;
+    require(!ctx->closed, NBODY_E_STATE, "context already destroyed");
+    if (ctx->children > 0) {
+      ctx->set_device();
+      hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+      ctx->closed = true;
+      return;
+    }
+    free_ctx(ctx);
   });
 }
 
@@ -248,6 +285,7 @@ int nbody_bodies_create(nbody_ctx* ctx, int64_t n, const double* masses,
     nbody::SystemParams p;
     p.n = n;
     b->id = b->eng->add(p, masses, pos_x, pos_y, pos_z, vel_x, vel_y, vel_z);
+    ++ctx->children;
     *out = b.release();
   });
 }
@@ -265,8 +303,10 @@ int nbody_bodies_download(nbody_ctx* ctx, nbody_bodies* b, double* masses,
 int nbody_bodies_destroy(nbody_bodies* b) {
   return guarded([&] {
     if (!b) return;
-    b->ctx->use();
+    nbody_ctx* ctx = b->ctx;
+    ctx->set_device();
     delete b;
+    release_child(ctx);
   });
 }
 
@@ -307,6 +347,32 @@ int nbody_step(nbody_ctx* ctx, nbody_bodies* b, int integrator,
     p.eps2 = eps2;
     b->eng->set_params(b->id, p);
     b->eng->step({b->id}, nsteps);
+  });
+}
+
+int nbody_step_record(nbody_ctx* ctx, nbody_bodies* b, int integrator,
+                      int gravity_method, int precision, double dt, double g,
+                      double eps2, int64_t nsteps, double* pos_x,
+                      double* pos_y, double* pos_z, int64_t max_stage_bytes) {
+  return guarded([&] {
+    require(ctx && b && b->ctx == ctx, NBODY_E_INVALID, "bad context/bodies");
+    check_gravity(gravity_method);
+    check_dt(dt);
+    require(nsteps >= 0, NBODY_E_INVALID, "nsteps < 0");
+    require(nsteps == 0 || (pos_x && pos_y && pos_z), NBODY_E_INVALID,
+            "null trajectory array");
+    require(max_stage_bytes >= 0, NBODY_E_INVALID, "max_stage_bytes < 0");
+    ctx->use();
+    b->eng->set_mode(mode_of(precision));
+    nbody::SystemParams p = b->eng->params(b->id);
+    p.integ = integ_of(integrator);
+    p.dt = dt;
+    p.g = g;
+    p.eps2 = eps2;
+    b->eng->set_params(b->id, p);
+    const size_t cap = max_stage_bytes > 0 ? (size_t)max_stage_bytes
+                                           : (size_t)64 << 20;
+    b->eng->record(b->id, nsteps, pos_x, pos_y, pos_z, cap);
   });
 }
 
@@ -382,6 +448,7 @@ int nbody_batch_create(nbody_ctx* ctx, int nsys, const int64_t* n,
                                      pos_z + o, vel_x + o, vel_y + o,
                                      vel_z + o));
     }
+    ++ctx->children;
     *out = bt.release();
   });
 }
@@ -425,8 +492,10 @@ int nbody_batch_energy(nbody_ctx* ctx, nbody_batch* bt, double* out) {
 int nbody_batch_destroy(nbody_batch* bt) {
   return guarded([&] {
     if (!bt) return;
-    bt->ctx->use();
+    nbody_ctx* ctx = bt->ctx;
+    ctx->set_device();
     delete bt;
+    release_child(ctx);
   });
 }
 
@@ -476,6 +545,72 @@ int nbody_server_stats(nbody_server* s, double out[8]) {
 int nbody_server_destroy(nbody_server* s) {
   return guarded([&] {
     if (s) nbody::server_destroy(reinterpret_cast<nbody::Server*>(s));
+  });
+}
+
+// ---- files ----------------------------------------------------------------
+int nbody_writer_open(const char* path, nbody_writer** out) {
+  return guarded([&] {
+    require(path && out, NBODY_E_INVALID, "null argument");
+    *out = nullptr;
+    auto w = std::make_unique<nbody_writer>();
+    w->w = std::make_unique<nbody::SnapshotWriter>(path);
+    *out = w.release();
+  });
+}
+
+int nbody_writer_append(nbody_writer* w, double time, int64_t n,
+                        const double* pos_x, const double* pos_y,
+                        const double* pos_z) {
+  return guarded([&] {
+    require(w != nullptr, NBODY_E_INVALID, "null writer");
+    w->w->append(time, n, pos_x, pos_y, pos_z);
+  });
+}
+
+int nbody_writer_close(nbody_writer* w) {
+  std::unique_ptr<nbody_writer> owned(w);
+  return guarded([&] {
+    if (owned) owned->w->close();
+  });
+}
+
+int nbody_format_f64(double v, char buf[32]) {
+  return guarded([&] {
+    require(buf != nullptr, NBODY_E_INVALID, "null buffer");
+    const int len = nbody::format_f64(v, buf);
+    buf[len] = '\0';
+  });
+}
+
+int nbody_parse_f64(const char* text, double* out) {
+  return guarded([&] {
+    require(text && out, NBODY_E_INVALID, "null argument");
+    require(nbody::parse_f64(text, std::strlen(text), out), NBODY_E_INVALID,
+            "invalid float literal");
+  });
+}
+
+int nbody_load_initial_conditions(const char* path, int64_t* n,
+                                  double* masses, double* pos_x,
+                                  double* pos_y, double* pos_z, double* vel_x,
+                                  double* vel_y, double* vel_z) {
+  return guarded([&] {
+    require(path && n, NBODY_E_INVALID, "null argument");
+    const nbody::InitialConditions ic = nbody::load_initial_conditions(path);
+    const int64_t count = (int64_t)ic.size();
+    double* dst[7] = {masses, pos_x, pos_y, pos_z, vel_x, vel_y, vel_z};
+    bool all = true;
+    for (double* d : dst) all = all && d != nullptr;
+    const int64_t cap = *n;
+    *n = count;
+    if (!all) return;
+    require(cap >= count, NBODY_E_NOSPACE,
+            "initial conditions hold more bodies than the arrays");
+    const std::vector<double>* src[7] = {&ic.m,  &ic.x,  &ic.y, &ic.z,
+                                         &ic.vx, &ic.vy, &ic.vz};
+    for (int c = 0; c < 7; ++c)
+      if (count) std::memcpy(dst[c], src[c]->data(), sizeof(double) * count);
   });
 }
 

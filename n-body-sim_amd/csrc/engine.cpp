@@ -55,6 +55,7 @@ Engine::~Engine() {
   hipFree(d_step_items_);
   hipFree(d_stage_);
   hipHostFree(h_stage_);
+  hipFree(d_traj_);
   hipFree(d_diag_);
   hipFree(d_list_);
   hipFree(d_sys_);
@@ -759,6 +760,67 @@ const double* Engine::gather(const std::vector<int>& ids, int nfields,
 
 void Engine::synchronize() {
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+// The step loop of step() with one gather launch after every step.  The
+// gather of step s (s counted within the current buffer fill) writes field
+// f at d_traj_[f * C * n + s * n + i], C = steps per fill, so each field of
+// a fill is one contiguous run of s-major rows and leaves in one copy.
+void Engine::record(int id, int64_t nsteps, double* x, double* y, double* z,
+                    size_t max_bytes) {
+  if (nsteps < 0) throw Error(NBODY_E_INVALID, "nsteps < 0");
+  check_id(id);
+  if (nsteps == 0) return;
+  if (!x || !y || !z) throw Error(NBODY_E_INVALID, "null trajectory array");
+  const int64_t n = sys_[id].p.n;
+  const int64_t per_step = 3 * n;  // doubles
+  int64_t C = (int64_t)(max_bytes / (sizeof(double) * (size_t)per_step));
+  C = std::max<int64_t>(1, std::min(C, nsteps));
+  if ((size_t)(C * per_step) > traj_cap_) {
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    hipFree(d_traj_);
+    d_traj_ = nullptr;
+    traj_cap_ = (size_t)(C * per_step);
+    hip_check(hipMalloc((void**)&d_traj_, sizeof(double) * traj_cap_),
+              "hipMalloc(trajectory)");
+  }
+  std::vector<WorkItem> gitems;
+  const int64_t tiles = (n + kWave - 1) / kWave;
+  for (int64_t t = 0; t < tiles; ++t) {
+    WorkItem w{id, (int32_t)t, (int16_t)STAGE_ACC, (int8_t)0, (int8_t)0};
+    w.pad_ = 0;
+    gitems.push_back(w);
+  }
+  prepare({id});
+  prime({id});
+  StepLists L;
+  upload_step_lists({id}, L);
+  validate_items(gitems);
+  ensure_items(gitems.size());
+  upload_table();
+  hip_check(hipMemcpyAsync(d_items_, gitems.data(),
+                           sizeof(WorkItem) * gitems.size(),
+                           hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(work list)");
+  double* dst[3] = {x, y, z};
+  int64_t done = 0;
+  while (done < nsteps) {
+    const int64_t c = std::min(C, nsteps - done);
+    for (int64_t s = 0; s < c; ++s) {
+      launch_step(L, (int)((done + s) & 1));
+      hip_check(launch_gather(d_arena_, d_items_, (int)gitems.size(), d_sys_,
+                              d_traj_ + s * n, C * n, 3, stream_),
+                "gather launch");
+    }
+    for (int f = 0; f < 3; ++f)
+      hip_check(hipMemcpyAsync(dst[f] + done * n, d_traj_ + f * C * n,
+                               sizeof(double) * (size_t)(c * n),
+                               hipMemcpyDeviceToHost, stream_),
+                "hipMemcpy(trajectory)");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    done += c;
+  }
+  finish_steps({id}, nsteps);
 }
 
 // The same loop as step(), with the launches of two consecutive steps

@@ -86,6 +86,14 @@ class Engine {
   // Graph-captured multi-step loop for a fixed set of systems (batched API).
   void step_graph(const std::vector<int>& ids, int64_t nsteps);
 
+  // Advances one system nsteps steps and copies its positions after every
+  // step to the host: step s of x at x[s*n .. s*n + n), likewise y and z.
+  // Positions are kept in a device trajectory buffer of at most
+  // max_bytes (at least one step) and copied out once per filled buffer.
+  // Synchronous.
+  void record(int id, int64_t nsteps, double* x, double* y, double* z,
+              size_t max_bytes);
+
   const Arena& arena() const { return A_; }
   hipStream_t stream() const { return stream_; }
 
@@ -149,6 +157,8 @@ class Engine {
   double* d_stage_ = nullptr;   // gather staging (device)
   double* h_stage_ = nullptr;   // gather staging (pinned host)
   size_t stage_cap_ = 0;        // doubles
+  double* d_traj_ = nullptr;    // trajectory buffer of record()
+  size_t traj_cap_ = 0;         // doubles
   std::vector<Sys> sys_;
   std::map<int64_t, int64_t> free_;  // offset -> length
   bool table_dirty_ = true;

@@ -69,6 +69,7 @@ def _bind(lib):
         "nbody_bodies_len": ([vp], i64),
         "nbody_calculate_accelerations": ([vp, vp, i, d, d, i, _D, _D, _D], i),
         "nbody_step": ([vp, vp, i, i, i, d, d, d, i64], i),
+        "nbody_step_record": ([vp, vp, i, i, i, d, d, d, i64, _D, _D, _D, i64], i),
         "nbody_energy_momentum": ([vp, vp, d, d, _D], i),
         "nbody_scenario_plummer": ([i64, u64, d, d, d] + [_D] * 7, i),
         "nbody_scenario_star_system": ([i64, u64, d, d, d, d, d, d, d] + [_D] * 7, i),
@@ -85,6 +86,12 @@ def _bind(lib):
         "nbody_server_cancel": ([vp, u64], i),
         "nbody_server_stats": ([vp, _D], i),
         "nbody_server_destroy": ([vp], i),
+        "nbody_writer_open": ([ctypes.c_char_p, pp], i),
+        "nbody_writer_append": ([vp, d, i64, _D, _D, _D], i),
+        "nbody_writer_close": ([vp], i),
+        "nbody_format_f64": ([d, ctypes.c_char_p], i),
+        "nbody_load_initial_conditions": ([ctypes.c_char_p, ctypes.POINTER(i64)]
+                                          + [_D] * 7, i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -387,12 +394,124 @@ class GpuSimulation:
     def snapshot(self) -> Bodies:
         return self.bodies.download()
 
+    def record(self, nsteps: int, max_stage_bytes: int = 0):
+        """Steps nsteps times keeping the positions after every step.
+
+        The SimulationRunner::run loop (src/simulation_runner.rs:28-52) of
+        step() + snapshot(), with the snapshots gathered on the device and
+        copied out in bulk.  Returns (times, x, y, z): times has nsteps
+        entries accumulated as ``time += dt``; x/y/z are (nsteps, n).
+        """
+        n = len(self.bodies)
+        out = [np.empty((nsteps, n)) for _ in range(3)]
+        _check(load_library().nbody_step_record(
+            self.gpu.handle, self.bodies.handle, int(self.integrator.integrator),
+            int(GravityMethod.NEWTON_PARALLEL), int(self.gravity.precision),
+            self.integrator.dt, self.gravity.g_constant, self.gravity.eps2,
+            int(nsteps), *[_p(a) for a in out], int(max_stage_bytes)))
+        times = np.empty(nsteps)
+        for k in range(nsteps):
+            self.time += self.dt
+            times[k] = self.time
+        return (times, *out)
+
     def energy_momentum(self):
         out = np.zeros(10)
         _check(load_library().nbody_energy_momentum(
             self.gpu.handle, self.bodies.handle, self.gravity.g_constant,
             self.gravity.eps2, _p(out)))
         return out
+
+
+# ---- files ------------------------------------------------------------------
+def load_bodies(path) -> Bodies:
+    """input::load_bodies (src/input.rs:20-33): CSV initial conditions."""
+    lib = load_library()
+    bpath = os.fsencode(path)
+    n = ctypes.c_int64(0)
+    _check(lib.nbody_load_initial_conditions(bpath, ctypes.byref(n), *([None] * 7)))
+    arrs = [np.empty(n.value) for _ in range(7)]
+    _check(lib.nbody_load_initial_conditions(bpath, ctypes.byref(n),
+                                             *[_p(a) for a in arrs]))
+    return Bodies(*arrs)
+
+
+def format_f64(v: float) -> str:
+    """A float as the snapshot CSV writes it (shortest round trip, ryu layout)."""
+    buf = ctypes.create_string_buffer(32)
+    _check(load_library().nbody_format_f64(float(v), buf))
+    return buf.value.decode()
+
+
+class SnapshotWriter:
+    """SimulationDataWriter (src/output.rs:73-139): .csv or .nbody by extension."""
+
+    def __init__(self, path):
+        h = ctypes.c_void_p()
+        _check(load_library().nbody_writer_open(os.fsencode(path), ctypes.byref(h)))
+        self._h = h
+
+    def append(self, time: float, x, y, z):
+        x, y, z = _f64(x), _f64(y), _f64(z)
+        if not (len(x) == len(y) == len(z)):
+            raise ValueError("x, y, z must have the same length")
+        _check(load_library().nbody_writer_append(self._h, float(time), len(x),
+                                                  _p(x), _p(y), _p(z)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            h, self._h = self._h, None
+            _check(load_library().nbody_writer_close(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+NBODY_MAGIC = b"NBODY001"
+NBODY_RECORD = np.dtype([("time", "<f8"), ("id", "<u8"), ("x", "<f8"),
+                         ("y", "<f8"), ("z", "<f8")])
+
+
+def read_snapshots(path):
+    """Reads a .nbody or .csv snapshot file into (times, ids, positions).
+
+    positions has shape (T, N, 3) in f64 -- the reshape of
+    gui/storage.py:140-168 / :106-137, without its f32 narrowing.
+    """
+    path = os.fspath(path)
+    if path.lower().endswith(".nbody"):
+        with open(path, "rb") as f:
+            if f.read(8) != NBODY_MAGIC:
+                raise ValueError("Invalid binary format: missing magic header")
+            arr = np.frombuffer(f.read(), dtype=NBODY_RECORD)
+        t, ids = arr["time"], arr["id"]
+        pos = np.stack((arr["x"], arr["y"], arr["z"]), axis=-1)
+    elif path.lower().endswith(".csv"):
+        with open(path, "rb") as f:
+            head = f.readline()
+        if not head:
+            return np.empty(0), np.empty(0, np.uint64), np.empty((0, 0, 3))
+        if head.strip() != b"time,id,x,y,z":
+            raise ValueError("unexpected CSV header")
+        raw = np.loadtxt(path, delimiter=",", skiprows=1, ndmin=2,
+                         dtype=np.float64)
+        t, ids, pos = raw[:, 0], raw[:, 1].astype(np.uint64), raw[:, 2:5]
+    else:
+        raise ValueError("Unsupported output file format")
+    ut = np.unique(t)
+    uid = np.unique(ids)
+    if len(t) != len(ut) * len(uid):
+        raise ValueError("records do not form a (time, id) grid")
+    return ut, uid, pos.reshape(len(ut), len(uid), 3)
 
 
 # ---- tuning -----------------------------------------------------------------

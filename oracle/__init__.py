@@ -64,6 +64,9 @@ def _load():
     lib.oracle_integrate.argtypes = [i, i64, d, d, d, i64, _D, _D, _D, _D, _D,
                                      _D, _D, i]
     lib.oracle_integrate.restype = ctypes.c_int
+    lib.oracle_integrate_record.argtypes = [i, i64, d, d, d, i64, _D, _D, _D,
+                                            _D, _D, _D, _D, _D, _D, _D, i]
+    lib.oracle_integrate_record.restype = ctypes.c_int
     lib.oracle_energy.argtypes = [i64, d, d, _D, _D, _D, _D, _D, _D, _D, _D]
     lib.oracle_energy.restype = None
     _lib = lib
@@ -141,6 +144,30 @@ def integrate(integrator, m, x, y, z, vx, vy, vz, g: float, softening: float,
     if rc != 0:
         raise ValueError("oracle_integrate failed")
     return tuple(st)
+
+
+def integrate_record(integrator, m, x, y, z, vx, vy, vz, g: float,
+                     softening: float, dt: float, nsteps: int,
+                     nthreads: int | None = None):
+    """Positions after every one of ``nsteps`` reference steps.
+
+    Returns (xs, ys, zs), each of shape (nsteps, n): row s holds the
+    positions after step s + 1, as the reference's run loop snapshots them.
+    """
+    lib = _load()
+    if isinstance(integrator, str):
+        integrator = INTEGRATORS[integrator]
+    m = _f64(m)
+    n = m.shape[0]
+    st = [np.array(a, dtype=np.float64, copy=True) for a in (x, y, z, vx, vy, vz)]
+    out = [np.empty((int(nsteps), n)) for _ in range(3)]
+    rc = lib.oracle_integrate_record(int(integrator), n, g, softening, dt,
+                                     int(nsteps), _p(m), *[_p(a) for a in st],
+                                     *[_p(a) for a in out],
+                                     nthreads or default_threads())
+    if rc != 0:
+        raise ValueError("oracle_integrate_record failed")
+    return tuple(out)
 
 
 def energy(m, x, y, z, vx, vy, vz, g: float, softening: float):

@@ -291,6 +291,28 @@ int oracle_integrate(int integrator, int64_t n, double g, double softening,
   return 0;
 }
 
+/* The run loop of SimulationRunner::run (src/simulation_runner.rs:28-52):
+ * one reference step at a time, positions copied after every step into
+ * out_x/out_y/out_z[s*n + i] (step s = 0 .. nsteps-1). */
+int oracle_integrate_record(int integrator, int64_t n, double g,
+                            double softening, double dt, int64_t nsteps,
+                            const double *m, double *rx, double *ry,
+                            double *rz, double *vx, double *vy, double *vz,
+                            double *out_x, double *out_y, double *out_z,
+                            int nthreads) {
+  for (int64_t s = 0; s < nsteps; ++s) {
+    int rc = oracle_integrate(integrator, n, g, softening, dt, 1, m, rx, ry,
+                              rz, vx, vy, vz, nthreads);
+    if (rc != 0) return rc;
+    for (int64_t i = 0; i < n; ++i) {
+      out_x[s * n + i] = rx[i];
+      out_y[s * n + i] = ry[i];
+      out_z[s * n + i] = rz[i];
+    }
+  }
+  return 0;
+}
+
 /* Softened total energy, not present in the reference (it has no energy
  * diagnostics): K = sum_i 0.5*m_i*|v_i|^2,
  * U = -g * sum_{i<j} m_i*m_j / sqrt(|r_ij|^2 + softening^2),

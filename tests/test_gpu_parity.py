@@ -210,8 +210,46 @@ def test_unsupported_gravity_methods(gpu):
     for gm in (nb.GravityMethod.NEWTON, nb.GravityMethod.BARNES_HUT):
         rc = lib.nbody_step(gpu.handle, db.handle, 0, int(gm), 1, 1e-3, 1.0, 0.0, 1)
         assert rc == nb.E_UNSUPPORTED
-    rc = lib.nbody_step(gpu.handle, db.handle, 0, 1, 1, 0.0, 1.0, 0.0, 1)
-    assert rc == nb.E_INVALID
+    for bad_dt in (float("nan"), float("inf")):
+        rc = lib.nbody_step(gpu.handle, db.handle, 0, 1, 1, bad_dt, 1.0, 0.0, 1)
+        assert rc == nb.E_INVALID
+    # the reference integrates any finite step: zero leaves the state in place
+    rc = lib.nbody_step(gpu.handle, db.handle, 0, 1, 1, 0.0, 1.0, 0.0, 3)
+    assert rc == nb.OK
+    after = db.download()
+    for f in b.fields():
+        assert np.array_equal(getattr(after, f), getattr(b, f))
+
+
+@pytest.mark.parametrize("integ", ["euler", "velocity-verlet", "runge-kutta"])
+def test_negative_time_step_strict(gpu, integ):
+    """A negative step runs backwards, as the reference's integrators do."""
+    b = nb.plummer(200, 4)
+    g, eps, dt, k = 1.0, 0.01, -1e-3, 6
+    ref = oracle.integrate(integ, *b.arrays(), g, eps, dt, k)
+    sim = nb.GpuSimulation(b, nb.SimulationParameters(g, dt, eps),
+                           integrator=nb.IntegratorMethod.from_str(integ),
+                           precision=nb.Precision.F64_STRICT)
+    sim.step(k)
+    out = sim.snapshot()
+    for q, f in enumerate(("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z")):
+        assert np.array_equal(getattr(out, f), ref[q]), f
+
+
+def test_context_destroyed_before_its_bodies(gpu):
+    """Destroy order is free: the context lives on until its last child."""
+    lib = nb.load_library()
+    ctx = nb.GpuContext(0)
+    b = nb.plummer(64, 2)
+    db = nb.DeviceBodies(ctx, b)
+    bt = nb.Batch(ctx, [b, b], [0, 1], 1e-3, 1.0, 0.01)
+    raw = ctx.handle
+    ctx.close()                                  # children still alive
+    rc = lib.nbody_step(raw, db.handle, 0, 1, 1, 1e-3, 1.0, 0.0, 1)
+    assert rc == nb.E_STATE
+    db.close()
+    bt.close()                                   # frees the context
+    assert lib.nbody_bodies_destroy(None) == nb.OK
 
 
 def _mixed_systems(count, seed):
