@@ -64,9 +64,21 @@ def _load():
     lib.oracle_integrate.argtypes = [i, i64, d, d, d, i64, _D, _D, _D, _D, _D,
                                      _D, _D, i]
     lib.oracle_integrate.restype = ctypes.c_int
-    lib.oracle_integrate_record.argtypes = [i, i64, d, d, d, i64, _D, _D, _D,
-                                            _D, _D, _D, _D, _D, _D, _D, i]
+    lib.oracle_integrate_gravity.argtypes = [i, i, d, i64, d, d, d, i64, _D, _D,
+                                             _D, _D, _D, _D, _D, i]
+    lib.oracle_integrate_gravity.restype = ctypes.c_int
+    lib.oracle_integrate_record.argtypes = [i, i, d, i64, d, d, d, i64, _D, _D,
+                                            _D, _D, _D, _D, _D, _D, _D, _D, i]
     lib.oracle_integrate_record.restype = ctypes.c_int
+    lib.oracle_barnes_hut.argtypes = [i64, d, d, d, _D, _D, _D, _D, _D, _D, _D, i]
+    lib.oracle_barnes_hut.restype = ctypes.c_int
+    lib.oracle_bh_morton.argtypes = [ctypes.c_uint64] * 3
+    lib.oracle_bh_morton.restype = ctypes.c_uint64
+    _I32 = ctypes.POINTER(ctypes.c_int32)
+    _U64 = ctypes.POINTER(ctypes.c_uint64)
+    lib.oracle_bh_tree.argtypes = [i64, _D, _D, _D, _D, _D, _D, _D, _D, _D,
+                                   _I64, _I64, _I32, _I64, _I64, _U64]
+    lib.oracle_bh_tree.restype = i64
     lib.oracle_energy.argtypes = [i64, d, d, _D, _D, _D, _D, _D, _D, _D, _D]
     lib.oracle_energy.restype = None
     _lib = lib
@@ -126,21 +138,31 @@ def newton_targets(m, x, y, z, g: float, softening: float, idx,
     return ax, ay, az
 
 
+GRAVITY = {"newton": 0, "newton-parallel": 1, "barnes-hut": 2}
+
+
+def _gravity_id(gravity) -> int:
+    return GRAVITY[gravity] if isinstance(gravity, str) else int(gravity)
+
+
 def integrate(integrator, m, x, y, z, vx, vy, vz, g: float, softening: float,
-              dt: float, nsteps: int, nthreads: int | None = None):
+              dt: float, nsteps: int, nthreads: int | None = None,
+              gravity: int | str = 1, theta: float = 0.5):
     """Advance copies of the state by ``nsteps`` reference steps.
 
-    ``integrator`` is 0/1/2 or "euler"/"velocity-verlet"/"runge-kutta".
-    Returns (x, y, z, vx, vy, vz) as new arrays.
+    ``integrator`` is 0/1/2 or "euler"/"velocity-verlet"/"runge-kutta";
+    ``gravity`` 0/1/2 or "newton"/"newton-parallel"/"barnes-hut" (with
+    ``theta``).  Returns (x, y, z, vx, vy, vz) as new arrays.
     """
     lib = _load()
     if isinstance(integrator, str):
         integrator = INTEGRATORS[integrator]
     m = _f64(m)
     st = [np.array(a, dtype=np.float64, copy=True) for a in (x, y, z, vx, vy, vz)]
-    rc = lib.oracle_integrate(int(integrator), m.shape[0], g, softening, dt,
-                              int(nsteps), _p(m), *[_p(a) for a in st],
-                              nthreads or default_threads())
+    rc = lib.oracle_integrate_gravity(int(integrator), _gravity_id(gravity), theta,
+                                      m.shape[0], g, softening, dt,
+                                      int(nsteps), _p(m), *[_p(a) for a in st],
+                                      nthreads or default_threads())
     if rc != 0:
         raise ValueError("oracle_integrate failed")
     return tuple(st)
@@ -148,7 +170,8 @@ def integrate(integrator, m, x, y, z, vx, vy, vz, g: float, softening: float,
 
 def integrate_record(integrator, m, x, y, z, vx, vy, vz, g: float,
                      softening: float, dt: float, nsteps: int,
-                     nthreads: int | None = None):
+                     nthreads: int | None = None, gravity: int | str = 1,
+                     theta: float = 0.5):
     """Positions after every one of ``nsteps`` reference steps.
 
     Returns (xs, ys, zs), each of shape (nsteps, n): row s holds the
@@ -161,13 +184,62 @@ def integrate_record(integrator, m, x, y, z, vx, vy, vz, g: float,
     n = m.shape[0]
     st = [np.array(a, dtype=np.float64, copy=True) for a in (x, y, z, vx, vy, vz)]
     out = [np.empty((int(nsteps), n)) for _ in range(3)]
-    rc = lib.oracle_integrate_record(int(integrator), n, g, softening, dt,
+    rc = lib.oracle_integrate_record(int(integrator), _gravity_id(gravity), theta,
+                                     n, g, softening, dt,
                                      int(nsteps), _p(m), *[_p(a) for a in st],
                                      *[_p(a) for a in out],
                                      nthreads or default_threads())
     if rc != 0:
         raise ValueError("oracle_integrate_record failed")
     return tuple(out)
+
+
+def barnes_hut(m, x, y, z, g: float, softening: float, theta: float,
+               nthreads: int | None = None):
+    """BarnesHutGravity::calculate_accelerations (one tree build + traversal)."""
+    lib = _load()
+    m, x, y, z = (_f64(a) for a in (m, x, y, z))
+    n = m.shape[0]
+    ax, ay, az = np.zeros(n), np.zeros(n), np.zeros(n)
+    rc = lib.oracle_barnes_hut(n, g, softening, theta, _p(m), _p(x), _p(y), _p(z),
+                               _p(ax), _p(ay), _p(az), nthreads or default_threads())
+    if rc != 0:
+        raise ValueError("oracle_barnes_hut failed")
+    return ax, ay, az
+
+
+def bh_tree(m, x, y, z):
+    """The octree of one build: dict of node arrays (preorder) and the
+    Morton order of the bodies (perm[k] = original index, codes[k])."""
+    lib = _load()
+    m, x, y, z = (_f64(a) for a in (m, x, y, z))
+    n = m.shape[0]
+    cap = n * 22 + 1
+    t = {k: np.zeros(cap) for k in ("mass", "com_x", "com_y", "com_z", "width")}
+    t["start"] = np.zeros(cap, np.int64)
+    t["count"] = np.zeros(cap, np.int64)
+    t["nchild"] = np.zeros(cap, np.int32)
+    t["child"] = np.zeros((cap, 8), np.int64)
+    perm = np.zeros(n, np.int64)
+    codes = np.zeros(n, np.uint64)
+    c = lib.oracle_bh_tree(n, _p(m), _p(x), _p(y), _p(z),
+                           *[_p(t[k]) for k in ("mass", "com_x", "com_y", "com_z",
+                                                "width")],
+                           t["start"].ctypes.data_as(_I64),
+                           t["count"].ctypes.data_as(_I64),
+                           t["nchild"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                           t["child"].ctypes.data_as(_I64),
+                           perm.ctypes.data_as(_I64),
+                           codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    if c < 0:
+        raise ValueError("oracle_bh_tree failed")
+    out = {k: v[:c] for k, v in t.items()}
+    out["perm"], out["codes"], out["node_count"] = perm, codes, int(c)
+    return out
+
+
+def morton(x: int, y: int, z: int) -> int:
+    return int(_load().oracle_bh_morton(x, y, z))
 
 
 def energy(m, x, y, z, vx, vy, vz, g: float, softening: float):

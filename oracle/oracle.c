@@ -132,11 +132,18 @@ void oracle_newton_targets(int64_t n, double g, double softening,
 }
 
 /* ---- integrators.  The force evaluation is newton (nthreads <= 1) or
- * newton-parallel; both give identical bits. */
+ * newton-parallel (both give identical bits), or barnes-hut (oracle_bh.c). */
 typedef struct {
   double g, softening;
   int nthreads;
+  int gravity; /* src/args.rs:84-89 order: 0/1 direct sum, 2 barnes-hut */
+  double theta;
 } grav_cfg;
+
+int oracle_barnes_hut(int64_t n, double g, double softening, double theta,
+                      const double *m, const double *rx, const double *ry,
+                      const double *rz, double *ax, double *ay, double *az,
+                      int nthreads);
 
 /* src/integrators/cpu/integrator.rs:29-44 (compute_acceleration): zero, then
  * evaluate the gravity method. */
@@ -146,7 +153,10 @@ static void eval_acc(const grav_cfg *c, int64_t n, const double *m,
   memset(ax, 0, sizeof(double) * n);
   memset(ay, 0, sizeof(double) * n);
   memset(az, 0, sizeof(double) * n);
-  if (c->nthreads <= 1)
+  if (c->gravity == 2)
+    oracle_barnes_hut(n, c->g, c->softening, c->theta, m, rx, ry, rz, ax, ay,
+                      az, c->nthreads);
+  else if (c->nthreads <= 1)
     oracle_newton(n, c->g, c->softening, m, rx, ry, rz, ax, ay, az);
   else
     oracle_newton_parallel(n, c->g, c->softening, m, rx, ry, rz, ax, ay, az,
@@ -271,12 +281,14 @@ static void rk4_step(const grav_cfg *c, int64_t n, double dt, const double *m,
  * (src/args.rs:140-144): 0 euler, 1 velocity-verlet, 2 runge-kutta.
  * Advances the bodies in place by nsteps steps.  Returns 0, or -1 on a bad
  * integrator id or allocation failure. */
-int oracle_integrate(int integrator, int64_t n, double g, double softening,
-                     double dt, int64_t nsteps, const double *m, double *rx,
-                     double *ry, double *rz, double *vx, double *vy,
-                     double *vz, int nthreads) {
+int oracle_integrate_gravity(int integrator, int gravity, double theta,
+                             int64_t n, double g, double softening, double dt,
+                             int64_t nsteps, const double *m, double *rx,
+                             double *ry, double *rz, double *vx, double *vy,
+                             double *vz, int nthreads) {
   if (integrator < 0 || integrator > 2 || n < 0 || nsteps < 0) return -1;
-  grav_cfg c = {g, softening, nthreads};
+  if (gravity < 0 || gravity > 2) return -1;
+  grav_cfg c = {g, softening, nthreads, gravity, theta};
   double *w = (double *)malloc(sizeof(double) * 27 * (size_t)(n > 0 ? n : 1));
   if (!w) return -1;
   for (int64_t s = 0; s < nsteps; ++s) {
@@ -291,18 +303,27 @@ int oracle_integrate(int integrator, int64_t n, double g, double softening,
   return 0;
 }
 
+int oracle_integrate(int integrator, int64_t n, double g, double softening,
+                     double dt, int64_t nsteps, const double *m, double *rx,
+                     double *ry, double *rz, double *vx, double *vy,
+                     double *vz, int nthreads) {
+  return oracle_integrate_gravity(integrator, 1, 0.0, n, g, softening, dt,
+                                  nsteps, m, rx, ry, rz, vx, vy, vz, nthreads);
+}
+
 /* The run loop of SimulationRunner::run (src/simulation_runner.rs:28-52):
  * one reference step at a time, positions copied after every step into
  * out_x/out_y/out_z[s*n + i] (step s = 0 .. nsteps-1). */
-int oracle_integrate_record(int integrator, int64_t n, double g,
-                            double softening, double dt, int64_t nsteps,
-                            const double *m, double *rx, double *ry,
-                            double *rz, double *vx, double *vy, double *vz,
-                            double *out_x, double *out_y, double *out_z,
-                            int nthreads) {
+int oracle_integrate_record(int integrator, int gravity, double theta,
+                            int64_t n, double g, double softening, double dt,
+                            int64_t nsteps, const double *m, double *rx,
+                            double *ry, double *rz, double *vx, double *vy,
+                            double *vz, double *out_x, double *out_y,
+                            double *out_z, int nthreads) {
   for (int64_t s = 0; s < nsteps; ++s) {
-    int rc = oracle_integrate(integrator, n, g, softening, dt, 1, m, rx, ry,
-                              rz, vx, vy, vz, nthreads);
+    int rc = oracle_integrate_gravity(integrator, gravity, theta, n, g,
+                                      softening, dt, 1, m, rx, ry, rz, vx, vy,
+                                      vz, nthreads);
     if (rc != 0) return rc;
     for (int64_t i = 0; i < n; ++i) {
       out_x[s * n + i] = rx[i];
