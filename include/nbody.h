@@ -44,7 +44,9 @@ extern "C" {
 /* GravityMethod, same order as the reference enum (src/args.rs:84-89). */
 #define NBODY_GRAVITY_NEWTON 0           /* not offered on GPU (src/gravity.rs:12) */
 #define NBODY_GRAVITY_NEWTON_PARALLEL 1  /* the GPU direct sum */
-#define NBODY_GRAVITY_BARNES_HUT 2       /* not offered on GPU (src/gravity.rs:14) */
+#define NBODY_GRAVITY_BARNES_HUT 2       /* the GPU octree, device bodies only
+                                            (the batched and served forms run
+                                            the direct sum) */
 /* IntegratorMethod (src/args.rs:140-144). */
 #define NBODY_INTEGRATOR_EULER 0
 #define NBODY_INTEGRATOR_VELOCITY_VERLET 1
@@ -107,11 +109,24 @@ int nbody_bodies_download(nbody_ctx *ctx, nbody_bodies *b, double *masses,
                           double *vel_x, double *vel_y, double *vel_z);
 int nbody_bodies_destroy(nbody_bodies *b);
 int64_t nbody_bodies_len(const nbody_bodies *b);
+/* Barnes-Hut opening angle used when gravity_method is
+ * NBODY_GRAVITY_BARNES_HUT: replaces the theta of BarnesHutGravity::new
+ * (src/gravity/cpu/barnes_hut.rs:15-21); default 0.5
+ * (src/constants.rs:21-22). */
+int nbody_bodies_set_theta(nbody_bodies *b, double theta);
+/* Octree nodes of the last Barnes-Hut build of these bodies (0 before any);
+ * the reference's node_count (octree.rs:353-357).  Synchronous. */
+int nbody_bodies_tree_nodes(nbody_bodies *b, int64_t *nodes);
 
 /* ---- gravity: replaces GpuGravity::calculate_accelerations
  * (src/gravity.rs:30-36; src/gravity/gpu/newton_parallel.rs:20-46).
- * gravity_method must be NBODY_GRAVITY_NEWTON_PARALLEL (others return
- * NBODY_E_UNSUPPORTED, where the reference panics, src/args.rs:117-120).
+ * gravity_method NBODY_GRAVITY_NEWTON_PARALLEL is the direct sum;
+ * NBODY_GRAVITY_BARNES_HUT is the reference's Barnes-Hut octree
+ * (src/gravity/cpu/barnes_hut.rs:25-51, rebuilt for every evaluation, with
+ * the bodies' theta; in F64_STRICT bit-identical to the reference's CPU
+ * Barnes-Hut, in the other precisions f64 with the fast inverse square root).
+ * NBODY_GRAVITY_NEWTON returns NBODY_E_UNSUPPORTED, where the reference
+ * panics (src/args.rs:117-120).
  * eps2 is the squared softening as the reference GPU path takes it
  * (src/args.rs:115).  Results stay in the device acceleration arrays
  * (DeviceBodies::acc_*); when acc_x/acc_y/acc_z are non-null they are also

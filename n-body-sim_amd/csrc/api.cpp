@@ -69,6 +69,7 @@ struct nbody_bodies {
   std::unique_ptr<Engine> eng;
   int id = -1;
   int64_t n = 0;
+  double theta = 0.5;  // src/constants.rs DEFAULT_THETA
 };
 
 struct nbody_writer {
@@ -124,6 +125,14 @@ void check_gravity(int gravity_method) {
 
 // Any finite step is accepted, as by the reference's integrators (a
 // negative step runs the system backwards, zero leaves it in place).
+// Device bodies also run Barnes-Hut (the reference has it on the CPU only,
+// src/gravity.rs:14); newton stays the CPU method as in gpu_create.
+int bodies_gravity(int gravity_method) {
+  if (gravity_method == NBODY_GRAVITY_BARNES_HUT) return nbody::GRAV_BARNES_HUT;
+  check_gravity(gravity_method);
+  return nbody::GRAV_DIRECT;
+}
+
 void check_dt(double dt) {
   require(std::isfinite(dt), NBODY_E_INVALID, "time step must be finite");
 }
@@ -312,6 +321,23 @@ int nbody_bodies_destroy(nbody_bodies* b) {
 
 int64_t nbody_bodies_len(const nbody_bodies* b) { return b ? b->n : 0; }
 
+int nbody_bodies_set_theta(nbody_bodies* b, double theta) {
+  return guarded([&] {
+    require(b != nullptr, NBODY_E_INVALID, "null bodies");
+    require(std::isfinite(theta) && theta >= 0.0, NBODY_E_INVALID,
+            "theta must be finite and >= 0");
+    b->theta = theta;
+  });
+}
+
+int nbody_bodies_tree_nodes(nbody_bodies* b, int64_t* nodes) {
+  return guarded([&] {
+    require(b && nodes, NBODY_E_INVALID, "null argument");
+    b->ctx->use();
+    *nodes = b->eng->tree_nodes(b->id);
+  });
+}
+
 // ---- gravity / integrator ----------------------------------------------
 int nbody_calculate_accelerations(nbody_ctx* ctx, nbody_bodies* b,
                                   int gravity_method, double g, double eps2,
@@ -319,9 +345,10 @@ int nbody_calculate_accelerations(nbody_ctx* ctx, nbody_bodies* b,
                                   double* acc_z) {
   return guarded([&] {
     require(ctx && b && b->ctx == ctx, NBODY_E_INVALID, "bad context/bodies");
-    check_gravity(gravity_method);
+    const int grav = bodies_gravity(gravity_method);
     ctx->use();
     b->eng->set_mode(mode_of(precision));
+    b->eng->set_gravity(grav, b->theta);
     nbody::SystemParams p = b->eng->params(b->id);
     p.g = g;
     p.eps2 = eps2;
@@ -335,11 +362,12 @@ int nbody_step(nbody_ctx* ctx, nbody_bodies* b, int integrator,
                double eps2, int64_t nsteps) {
   return guarded([&] {
     require(ctx && b && b->ctx == ctx, NBODY_E_INVALID, "bad context/bodies");
-    check_gravity(gravity_method);
+    const int grav = bodies_gravity(gravity_method);
     check_dt(dt);
     require(nsteps >= 0, NBODY_E_INVALID, "nsteps < 0");
     ctx->use();
     b->eng->set_mode(mode_of(precision));
+    b->eng->set_gravity(grav, b->theta);
     nbody::SystemParams p = b->eng->params(b->id);
     p.integ = integ_of(integrator);
     p.dt = dt;
@@ -356,7 +384,7 @@ int nbody_step_record(nbody_ctx* ctx, nbody_bodies* b, int integrator,
                       double* pos_y, double* pos_z, int64_t max_stage_bytes) {
   return guarded([&] {
     require(ctx && b && b->ctx == ctx, NBODY_E_INVALID, "bad context/bodies");
-    check_gravity(gravity_method);
+    const int grav = bodies_gravity(gravity_method);
     check_dt(dt);
     require(nsteps >= 0, NBODY_E_INVALID, "nsteps < 0");
     require(nsteps == 0 || (pos_x && pos_y && pos_z), NBODY_E_INVALID,
@@ -364,6 +392,7 @@ int nbody_step_record(nbody_ctx* ctx, nbody_bodies* b, int integrator,
     require(max_stage_bytes >= 0, NBODY_E_INVALID, "max_stage_bytes < 0");
     ctx->use();
     b->eng->set_mode(mode_of(precision));
+    b->eng->set_gravity(grav, b->theta);
     nbody::SystemParams p = b->eng->params(b->id);
     p.integ = integ_of(integrator);
     p.dt = dt;

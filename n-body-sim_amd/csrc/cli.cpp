@@ -10,11 +10,12 @@
 //
 // Every force evaluation runs on the GPU.  Without --gpu the reference runs
 // its f64 CPU code; here that selection runs the GPU in the F64_STRICT
-// arithmetic, whose results are bit-identical to that CPU code for newton
-// and newton-parallel, so the output files are identical byte for byte.
-// With --gpu the reference's GPU kernel (rsqrt, contracted FMAs) is
-// replaced by the fast F64 arithmetic, for all three integrators (the
-// reference offers Euler only there).  --precision overrides either choice.
+// arithmetic, whose results are bit-identical to that CPU code for newton,
+// newton-parallel and barnes-hut, so the output files are identical byte for
+// byte.  With --gpu the reference's GPU kernel (rsqrt, contracted FMAs) is
+// replaced by the fast F64 arithmetic, for all three integrators and for
+// barnes-hut too (the reference offers newton-parallel with Euler only
+// there).  --precision overrides either choice.
 #include <algorithm>
 #include <condition_variable>
 #include <cstdio>
@@ -81,7 +82,8 @@ void print_help() {
       "      --progress\n"
       "          Flag to enable percent progress output to stdout\n"
       "      --gpu\n"
-      "          Flag to select the fast GPU arithmetic (all integrators)\n"
+      "          Flag to select the fast GPU arithmetic (all integrators, "
+      "newton-parallel and barnes-hut)\n"
       "      --precision <PRECISION>\n"
       "          Force arithmetic [default: f64-strict, f64 with --gpu] "
       "[possible values: f64-strict, f64, f32c, f32r]\n"
@@ -321,18 +323,18 @@ std::string method_name(const std::string& s) {  // Debug form of the enum
 int main(int argc, char** argv) {
   const Args a = parse_args(argc, argv);
 
-  // Method selection first: src/args.rs:111-123 panics for anything but
-  // newton-parallel on the GPU (exit status 101 from the panic).
-  if (a.gpu && a.gravity != "newton-parallel") {
+  // Method selection first: src/args.rs:111-123 panics for newton on the
+  // GPU (exit status 101 from the panic).  newton and newton-parallel are the
+  // same direct sum (bit for bit); barnes-hut is the octree method.
+  if (a.gpu && a.gravity == "newton") {
     std::fprintf(stderr,
                  "GPU-accelerated version of %s gravity method is not "
                  "implemented yet\n",
                  method_name(a.gravity).c_str());
     return 101;
   }
-  if (a.gravity == "barnes-hut")
-    fail("the barnes-hut gravity method is not available in this build; "
-         "use newton or newton-parallel");
+  const int gravity = a.gravity == "barnes-hut" ? NBODY_GRAVITY_BARNES_HUT
+                                                 : NBODY_GRAVITY_NEWTON_PARALLEL;
 
   // Initial conditions (src/input.rs:20-33).
   int64_t n = 0;
@@ -395,7 +397,8 @@ int main(int argc, char** argv) {
     if (nbody_ctx_create(a.device, &ctx) != 0 ||
         nbody_bodies_create(ctx, n, ic[0].data(), ic[1].data(), ic[2].data(),
                             ic[3].data(), ic[4].data(), ic[5].data(),
-                            ic[6].data(), &bodies) != 0) {
+                            ic[6].data(), &bodies) != 0 ||
+        nbody_bodies_set_theta(bodies, a.theta) != 0) {
       sim_error = last_error();
       return finish(1);
     }
@@ -428,8 +431,7 @@ int main(int argc, char** argv) {
     b->y.resize(k * (uint64_t)n);
     b->z.resize(k * (uint64_t)n);
     if (n > 0 &&
-        nbody_step_record(ctx, bodies, integ, NBODY_GRAVITY_NEWTON_PARALLEL,
-                          precision, a.time_step, a.g_constant, eps2,
+        nbody_step_record(ctx, bodies, integ, gravity, precision, a.time_step, a.g_constant, eps2,
                           (int64_t)k, b->x.data(), b->y.data(), b->z.data(),
                           0) != 0) {
       sim_error = last_error();

@@ -1,6 +1,8 @@
 // Host engine implementation (see engine.h).
 #include "engine.h"
 
+#include "bh.h"
+
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -271,6 +273,7 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
 void Engine::remove(int id) {
   step_sig_.clear();
   check_id(id);
+  trees_.erase(id);
   Sys& s = sys_[id];
   // Clear the range so stale values can never be read as sources.
   for (double* a : {A_.x, A_.y, A_.z, A_.sx[0], A_.sy[0], A_.sz[0], A_.sx[1],
@@ -332,6 +335,48 @@ void Engine::set_mode(int mode) {
   }
   table_dirty_ = true;  // split counts depend on the mode
   drop_graphs();
+}
+
+void Engine::set_gravity(int gravity, double theta) {
+  if (gravity != GRAV_DIRECT && gravity != GRAV_BARNES_HUT)
+    throw Error(NBODY_E_INVALID, "unknown gravity method");
+  if (gravity == GRAV_BARNES_HUT && !(std::isfinite(theta) && theta >= 0.0))
+    throw Error(NBODY_E_INVALID, "theta must be finite and >= 0");
+  if (gravity == gravity_ && (gravity == GRAV_DIRECT || theta == theta_)) return;
+  gravity_ = gravity;
+  theta_ = theta;
+  step_sig_.clear();
+  for (auto& s : sys_) {
+    s.vv_primed = false;
+    s.prepared = false;
+  }
+  drop_graphs();
+}
+
+void Engine::bh_round(int id, int stage, int src) {
+  upload_table();
+  std::unique_ptr<BarnesHut>& t = trees_[id];
+  if (!t) t.reset(new BarnesHut(sys_[id].p.n));
+  t->evaluate(stream_, mode_ == MODE_F64_STRICT, d_arena_, d_sys_, id,
+              h_sys_[id], stage, src, theta_);
+}
+
+void Engine::bh_step(const std::vector<int>& ids, int64_t t) {
+  for (int id : ids) {
+    const Sys& s = sys_[id];
+    if (s.p.integ == INTEG_RK4) {
+      for (int r = 0; r < 4; ++r) bh_round(id, STAGE_RK0 + r, s.cur ^ (r & 1));
+    } else {
+      bh_round(id, s.p.integ == INTEG_EULER ? STAGE_EULER : STAGE_VV,
+               s.cur ^ (int)(t & 1));
+    }
+  }
+}
+
+int64_t Engine::tree_nodes(int id) {
+  check_id(id);
+  auto it = trees_.find(id);
+  return it == trees_.end() ? 0 : it->second->node_count(stream_);
 }
 
 void Engine::set_origin(Sys& s, SysDesc& d) const {
@@ -524,11 +569,13 @@ void Engine::prime(const std::vector<int>& ids) {
   std::vector<WorkItem> items, combine;
   for (int id : ids) {
     const Sys& s = sys_[id];
-    if (s.p.integ == INTEG_VV && !s.vv_primed)
+    if (s.p.integ != INTEG_VV || s.vv_primed) continue;
+    if (gravity_ == GRAV_BARNES_HUT)
+      bh_round(id, STAGE_VV_PRIME, s.cur);
+    else
       tiles_of(id, STAGE_VV_PRIME, s.cur, items, &combine);
   }
-  if (items.empty()) return;
-  run_round(items, combine);
+  if (!items.empty()) run_round(items, combine);
   for (int id : ids) {
     Sys& s = sys_[id];
     if (s.p.integ == INTEG_VV && !s.vv_primed) {
@@ -622,6 +669,11 @@ void Engine::step(const std::vector<int>& ids, int64_t nsteps) {
   if (nsteps == 0 || ids.empty()) return;
   prepare(ids);
   prime(ids);
+  if (gravity_ == GRAV_BARNES_HUT) {
+    for (int64_t t = 0; t < nsteps; ++t) bh_step(ids, t);
+    finish_steps(ids, nsteps);
+    return;
+  }
   StepLists L;
   upload_step_lists(ids, L);
   for (int64_t t = 0; t < nsteps; ++t) launch_step(L, (int)(t & 1));
@@ -636,9 +688,13 @@ void Engine::accelerations(int id, double* ax, double* ay, double* az) {
     s.prepared = false;
   }
   prepare({id});
-  std::vector<WorkItem> items, combine;
-  tiles_of(id, STAGE_ACC, s.cur, items, &combine);
-  run_round(items, combine);
+  if (gravity_ == GRAV_BARNES_HUT) {
+    bh_round(id, STAGE_ACC, s.cur);
+  } else {
+    std::vector<WorkItem> items, combine;
+    tiles_of(id, STAGE_ACC, s.cur, items, &combine);
+    run_round(items, combine);
+  }
   s.vv_primed = false;  // the carried acceleration was overwritten
   if (ax || ay || az) {
     const size_t bytes = sizeof(double) * s.p.n;
@@ -793,8 +849,9 @@ void Engine::record(int id, int64_t nsteps, double* x, double* y, double* z,
   }
   prepare({id});
   prime({id});
+  const bool bh = gravity_ == GRAV_BARNES_HUT;
   StepLists L;
-  upload_step_lists({id}, L);
+  if (!bh) upload_step_lists({id}, L);
   validate_items(gitems);
   ensure_items(gitems.size());
   upload_table();
@@ -807,7 +864,10 @@ void Engine::record(int id, int64_t nsteps, double* x, double* y, double* z,
   while (done < nsteps) {
     const int64_t c = std::min(C, nsteps - done);
     for (int64_t s = 0; s < c; ++s) {
-      launch_step(L, (int)((done + s) & 1));
+      if (bh)
+        bh_step({id}, done + s);
+      else
+        launch_step(L, (int)((done + s) & 1));
       hip_check(launch_gather(d_arena_, d_items_, (int)gitems.size(), d_sys_,
                               d_traj_ + s * n, C * n, 3, stream_),
                 "gather launch");
@@ -830,6 +890,8 @@ void Engine::record(int id, int64_t nsteps, double* x, double* y, double* z,
 // contents are re-uploaded on every call.
 void Engine::step_graph(const std::vector<int>& ids, int64_t nsteps) {
   if (nsteps < 0) throw Error(NBODY_E_INVALID, "nsteps < 0");
+  if (gravity_ != GRAV_DIRECT)
+    throw Error(NBODY_E_UNSUPPORTED, "graph stepping runs the direct sum only");
   for (int id : ids) check_id(id);
   if (nsteps == 0 || ids.empty()) return;
   prepare(ids);

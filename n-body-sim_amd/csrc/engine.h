@@ -12,6 +12,7 @@
 
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -19,6 +20,8 @@
 #include "nbody_types.h"
 
 namespace nbody {
+
+class BarnesHut;
 
 // Error carrying a public NBODY_E_* code.
 struct Error : std::runtime_error {
@@ -46,6 +49,14 @@ class Engine {
   int mode() const { return mode_; }
   // Changing the arithmetic rebuilds every snapshot and drops carried state.
   void set_mode(int mode);
+  // Force law (GRAV_*) and Barnes-Hut opening angle.  A change drops
+  // carried accelerations and rebuilds the snapshots, as a mode change does.
+  // Barnes-Hut evaluates in f64: the strict arithmetic in MODE_F64_STRICT,
+  // the fast f64 arithmetic in every other mode.
+  void set_gravity(int gravity, double theta);
+  int gravity() const { return gravity_; }
+  // Octree nodes of a system's last Barnes-Hut build (synchronises).
+  int64_t tree_nodes(int id);
 
   // Uploads a system; returns its id.  Throws Error(NBODY_E_NOSPACE) when no
   // contiguous range of the arena is free.
@@ -138,10 +149,17 @@ class Engine {
   void launch_step(const StepLists& L, int which);
   void finish_steps(const std::vector<int>& ids, int64_t nsteps);
   void check_id(int id) const;
+  // One Barnes-Hut force evaluation of a system with its stage epilogue.
+  void bh_round(int id, int stage, int src);
+  // One step of every listed system with Barnes-Hut forces (parity t & 1).
+  void bh_step(const std::vector<int>& ids, int64_t t);
 
   int device_;
   hipStream_t stream_;
   int mode_;
+  int gravity_ = GRAV_DIRECT;
+  double theta_ = 0.5;
+  std::map<int, std::unique_ptr<BarnesHut>> trees_;
   int64_t cap_;
   int max_sys_;
   Arena A_{};

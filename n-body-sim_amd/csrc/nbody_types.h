@@ -42,6 +42,10 @@ enum Mode : int32_t {            // force arithmetic (public NBODY_PRECISION_*)
 
 enum Integ : int32_t { INTEG_EULER = 0, INTEG_VV = 1, INTEG_RK4 = 2 };
 
+// Force law of an engine's systems: the direct sum, or the Barnes-Hut octree
+// (one tree per system, rebuilt for every evaluation).
+enum Gravity : int32_t { GRAV_DIRECT = 0, GRAV_BARNES_HUT = 1 };
+
 // What the epilogue of one force evaluation does for a system.
 enum Stage : int32_t {
   STAGE_ACC = 0,       // store a into ax/ay/az; state unchanged

@@ -67,6 +67,8 @@ def _bind(lib):
         "nbody_bodies_download": ([vp, vp] + [_D] * 7, i),
         "nbody_bodies_destroy": ([vp], i),
         "nbody_bodies_len": ([vp], i64),
+        "nbody_bodies_set_theta": ([vp, d], i),
+        "nbody_bodies_tree_nodes": ([vp, ctypes.POINTER(i64)], i),
         "nbody_calculate_accelerations": ([vp, vp, i, d, d, i, _D, _D, _D], i),
         "nbody_step": ([vp, vp, i, i, i, d, d, d, i64], i),
         "nbody_step_record": ([vp, vp, i, i, i, d, d, d, i64, _D, _D, _D, i64], i),
@@ -170,14 +172,16 @@ class GravityMethod(enum.IntEnum):
             raise ValueError(f"Invalid gravity method: {s}") from None
 
     def gpu_create(self, params: "SimulationParameters",
-                   precision: Precision = Precision.F64) -> "GpuNewtonParallelGravity":
-        # src/args.rs:111-123: only newton-parallel runs on the GPU; eps2 = eps^2
-        if self is not GravityMethod.NEWTON_PARALLEL:
-            raise NbodyError(E_UNSUPPORTED, "GPU-accelerated version of "
-                             f"{self.name} gravity method is not implemented")
-        return GpuNewtonParallelGravity(params.g_constant,
-                                        params.softening_factor * params.softening_factor,
-                                        precision)
+                   precision: Precision = Precision.F64):
+        # src/args.rs:111-123 (newton-parallel only there; barnes-hut added
+        # here); eps2 = eps^2
+        eps2 = params.softening_factor * params.softening_factor
+        if self is GravityMethod.NEWTON_PARALLEL:
+            return GpuNewtonParallelGravity(params.g_constant, eps2, precision)
+        if self is GravityMethod.BARNES_HUT:
+            return GpuBarnesHutGravity(params.g_constant, eps2, params.theta, precision)
+        raise NbodyError(E_UNSUPPORTED, "GPU-accelerated version of "
+                         f"{self.name} gravity method is not implemented")
 
 
 class IntegratorMethod(enum.IntEnum):
@@ -208,6 +212,7 @@ class SimulationParameters:
     g_constant: float = 1.0
     time_step: float = 0.01
     softening_factor: float = 0.005
+    theta: float = 0.5
 
 
 @dataclass
@@ -328,21 +333,49 @@ class DeviceBodies:
 class GpuNewtonParallelGravity:
     """src/gravity/gpu/newton_parallel.rs: direct sum on the GPU."""
 
+    method = GravityMethod.NEWTON_PARALLEL
+
     def __init__(self, g_constant: float, eps2: float,
                  precision: Precision = Precision.F64):
         self.g_constant = float(g_constant)
         self.eps2 = float(eps2)
         self.precision = Precision(precision)
 
+    def configure(self, bodies: "DeviceBodies"):
+        """Per-method settings of the bodies (none for the direct sum)."""
+
     def calculate_accelerations(self, gpu: GpuContext, bodies: DeviceBodies,
                                 download: bool = True):
         """Accelerations into the device arrays; returned as (ax, ay, az)."""
         n = len(bodies)
         out = [np.empty(n) for _ in range(3)] if download else [None] * 3
+        self.configure(bodies)
         _check(load_library().nbody_calculate_accelerations(
-            gpu.handle, bodies.handle, int(GravityMethod.NEWTON_PARALLEL),
+            gpu.handle, bodies.handle, int(self.method),
             self.g_constant, self.eps2, int(self.precision), *[_p(a) for a in out]))
         return tuple(out) if download else None
+
+
+class GpuBarnesHutGravity(GpuNewtonParallelGravity):
+    """src/gravity/cpu/barnes_hut.rs on the GPU: the reference's octree,
+    rebuilt for every evaluation (bit-identical in Precision.F64_STRICT)."""
+
+    method = GravityMethod.BARNES_HUT
+
+    def __init__(self, g_constant: float, eps2: float, theta: float = 0.5,
+                 precision: Precision = Precision.F64):
+        super().__init__(g_constant, eps2, precision)
+        self.theta = float(theta)
+
+    def configure(self, bodies: "DeviceBodies"):
+        _check(load_library().nbody_bodies_set_theta(bodies.handle, self.theta))
+
+    @staticmethod
+    def tree_nodes(bodies: "DeviceBodies") -> int:
+        """Octree nodes of the last build."""
+        c = ctypes.c_int64()
+        _check(load_library().nbody_bodies_tree_nodes(bodies.handle, ctypes.byref(c)))
+        return c.value
 
 
 class GpuIntegrator:
@@ -354,9 +387,10 @@ class GpuIntegrator:
     def step(self, gpu: GpuContext, bodies: DeviceBodies,
              gravity: GpuNewtonParallelGravity, nsteps: int = 1):
         """GpuIntegrator::step, nsteps times in one call."""
+        gravity.configure(bodies)
         _check(load_library().nbody_step(
             gpu.handle, bodies.handle, int(self.integrator),
-            int(GravityMethod.NEWTON_PARALLEL), int(gravity.precision), self.dt,
+            int(gravity.method), int(gravity.precision), self.dt,
             gravity.g_constant, gravity.eps2, int(nsteps)))
 
 
@@ -404,9 +438,10 @@ class GpuSimulation:
         """
         n = len(self.bodies)
         out = [np.empty((nsteps, n)) for _ in range(3)]
+        self.gravity.configure(self.bodies)
         _check(load_library().nbody_step_record(
             self.gpu.handle, self.bodies.handle, int(self.integrator.integrator),
-            int(GravityMethod.NEWTON_PARALLEL), int(self.gravity.precision),
+            int(self.gravity.method), int(self.gravity.precision),
             self.integrator.dt, self.gravity.g_constant, self.gravity.eps2,
             int(nsteps), *[_p(a) for a in out], int(max_stage_bytes)))
         times = np.empty(nsteps)

@@ -145,10 +145,13 @@ def test_gpu_create_selection():
     p = nb.SimulationParameters(g_constant=2.0, time_step=0.01, softening_factor=0.1)
     grav = nb.GravityMethod.NEWTON_PARALLEL.gpu_create(p)
     assert grav.g_constant == 2.0 and grav.eps2 == 0.1 * 0.1   # src/args.rs:115
-    for gm in (nb.GravityMethod.NEWTON, nb.GravityMethod.BARNES_HUT):
-        with pytest.raises(nb.NbodyError) as e:
-            gm.gpu_create(p)
-        assert e.value.code == nb.E_UNSUPPORTED
+    with pytest.raises(nb.NbodyError) as e:
+        nb.GravityMethod.NEWTON.gpu_create(p)
+    assert e.value.code == nb.E_UNSUPPORTED
+    p.theta = 0.7
+    bh = nb.GravityMethod.BARNES_HUT.gpu_create(p, nb.Precision.F64_STRICT)
+    assert isinstance(bh, nb.GpuBarnesHutGravity) and bh.theta == 0.7
+    assert bh.method == nb.GravityMethod.BARNES_HUT and bh.eps2 == 0.1 * 0.1
     for im, cls in ((nb.IntegratorMethod.EULER, nb.GpuEulerIntegrator),
                     (nb.IntegratorMethod.VELOCITY_VERLET, nb.GpuVelocityVerletIntegrator),
                     (nb.IntegratorMethod.RUNGE_KUTTA, nb.GpuRungeKuttaIntegrator)):

@@ -207,9 +207,10 @@ def test_unsupported_gravity_methods(gpu):
     b = nb.plummer(8, 1)
     db = nb.DeviceBodies(gpu, b)
     lib = nb.load_library()
-    for gm in (nb.GravityMethod.NEWTON, nb.GravityMethod.BARNES_HUT):
-        rc = lib.nbody_step(gpu.handle, db.handle, 0, int(gm), 1, 1e-3, 1.0, 0.0, 1)
-        assert rc == nb.E_UNSUPPORTED
+    rc = lib.nbody_step(gpu.handle, db.handle, 0, int(nb.GravityMethod.NEWTON), 1,
+                        1e-3, 1.0, 0.0, 1)
+    assert rc == nb.E_UNSUPPORTED
+    assert lib.nbody_step(gpu.handle, db.handle, 0, 7, 1, 1e-3, 1.0, 0.0, 1) == nb.E_INVALID
     for bad_dt in (float("nan"), float("inf")):
         rc = lib.nbody_step(gpu.handle, db.handle, 0, 1, 1, bad_dt, 1.0, 0.0, 1)
         assert rc == nb.E_INVALID
