@@ -10,8 +10,10 @@ of the whole system; interactions counted as N(N-1) per force evaluation and
 the timed region is bracketed by HIP events on the library's stream.
 
 Other workloads: --workload batched (512 star systems, N log-uniform in
-[64, 8192], Euler/VV/RK4 1:1:1, dt=1e-3) and --workload large (two colliding
-Plummer spheres, N=1,048,576, RK4).
+[64, 8192], Euler/VV/RK4 1:1:1, dt=1e-3), --workload large (two colliding
+Plummer spheres, N=1,048,576, RK4), --workload serve (the request server
+under Poisson arrivals) and --workload bh (Barnes-Hut, theta 0.5, on the
+N=1,048,576 system).
 
 cpu_baseline: the oracle's threaded direct sum (the reference's
 newton-parallel restated in C) on this host, full N, repeated for ~10 s.
@@ -346,6 +348,103 @@ def run_large(args, gpu):
     }
 
 
+def run_bh(args, gpu):
+    """Barnes-Hut, theta 0.5, on the N=1,048,576 colliding spheres: the tree
+    is rebuilt for every evaluation, as the reference does.  Timed: fast-f64
+    RK4 steps (4 evaluations each) and single strict evaluations; beside
+    them one f64 direct-sum evaluation of the same system for the speedup.
+    Parity at full size: strict forces against the oracle's Barnes-Hut on
+    4096 sampled targets, bit for bit."""
+    n, g, eps, dt, theta = 1 << 20, 1.0, 0.002, 2.5e-4, 0.5
+    b = nb.colliding(n, 5, separation=10.0, rel_speed=0.5)
+    strict = nb.GpuBarnesHutGravity(g, eps * eps, theta, nb.Precision.F64_STRICT)
+    fast = nb.GpuBarnesHutGravity(g, eps * eps, theta, nb.Precision.F64)
+    db = nb.DeviceBodies(gpu, b)
+    acc_strict = strict.calculate_accelerations(gpu, db)      # warm-up + parity
+    nodes = strict.tree_nodes(db)
+    times = {}
+    for name, grav in (("strict", strict), ("fast", fast)):
+        grav.calculate_accelerations(gpu, db, download=False)
+        gpu.synchronize()
+        gpu.record(0)
+        for _ in range(3):
+            grav.calculate_accelerations(gpu, db, download=False)
+        gpu.record(1)
+        times[name] = gpu.elapsed_ms(0, 1) / 3
+        log(f"bh {name}: {times[name]:.2f} ms per evaluation")
+    acc_fast = fast.calculate_accelerations(gpu, db)
+    direct = nb.GpuNewtonParallelGravity(g, eps * eps, nb.Precision.F64)
+    gpu.record(0)
+    acc_direct = direct.calculate_accelerations(gpu, db)
+    gpu.record(1)
+    direct_ms = gpu.elapsed_ms(0, 1)
+    db.close()
+    # RK4 steps with fast Barnes-Hut forces
+    db = nb.DeviceBodies(gpu, b)
+    rk = nb.GpuRungeKuttaIntegrator(dt)
+    e0 = energy(gpu, db, g, eps)
+    rk.step(gpu, db, fast, 1)
+    gpu.synchronize()
+    gpu.record(0)
+    rk.step(gpu, db, fast, args.steps)
+    gpu.record(1)
+    step_ms = gpu.elapsed_ms(0, 1) / args.steps
+    e1 = energy(gpu, db, g, eps)
+    db.close()
+    acc = {"fast_vs_direct_max_rel": None, "fast_vs_direct_median_rel": None}
+    mx, med = rel_err(acc_fast, acc_direct)
+    acc.update(fast_vs_direct_max_rel=mx, fast_vs_direct_median_rel=med,
+               strict_vs_direct_median_rel=rel_err(acc_strict, acc_direct)[1],
+               energy_drift_rk4=abs(e1[2] - e0[2]) / abs(e0[2]))
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        idx = np.sort(np.random.default_rng(9).choice(n, 4096, replace=False))
+        threads = min(16, oracle.default_threads())
+        t0 = time.perf_counter()
+        ref = oracle.barnes_hut_targets(b.masses, b.pos_x, b.pos_y, b.pos_z, g, eps,
+                                        theta, idx, threads)
+        t_sample = time.perf_counter() - t0
+        same = int(sum(np.array_equal(np.array([a[i] for a in acc_strict]),
+                                      np.array([r[q] for r in ref]))
+                       for q, i in enumerate(idx)))
+        acc["strict_bitwise_sampled"] = f"{same}/4096"
+        mx_f, _ = rel_err(tuple(a[idx] for a in acc_fast), ref)
+        acc["fast_vs_oracle_max_rel_sampled"] = mx_f
+        # CPU baseline: the oracle's full evaluation (tree + every target)
+        t0 = time.perf_counter()
+        oracle.barnes_hut(b.masses, b.pos_x, b.pos_y, b.pos_z, g, eps, theta, threads)
+        t_full = time.perf_counter() - t0
+        log(f"oracle barnes-hut: sample {t_sample:.1f} s, full {t_full:.1f} s")
+        cpu = {"value": 1.0 / t_full, "unit": "evaluations/s", "cores": threads,
+               "kind": "port",
+               "sample": f"one full evaluation of the same N={n} system (oracle "
+                         f"barnes-hut: tree build + all targets, {threads} threads, "
+                         f"{t_full:.1f} s)"}
+    evals_per_s = 1e3 / times["fast"]
+    return {
+        "metric": "barnes-hut force evaluations/s (theta 0.5, tree rebuilt per evaluation)",
+        "value": evals_per_s, "unit": "evaluations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": step_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (two seeded Plummer spheres)",
+        "config": {"workload": "barnes-hut theta 0.5 on two colliding Plummer spheres "
+                               "N=1048576, softening 0.002; RK4 dt 2.5e-4", "n": n,
+                   "parallelism": "none (1 GPU)"},
+        "roofline": None,
+        "roofline_note": "tree walk: latency-bound pointer chasing through "
+                         "scalar-cached node data; no VALU or HBM roofline applies",
+        "cpu_baseline": cpu,
+        "bh": {"fast_ms_per_eval": times["fast"], "strict_ms_per_eval": times["strict"],
+               "direct_f64_ms_per_eval": direct_ms,
+               "speedup_vs_direct_f64": direct_ms / times["fast"],
+               "direct_equivalent_interactions_per_s": n * (n - 1) / (times["fast"] / 1e3),
+               "tree_nodes": nodes},
+        "accuracy": acc,
+    }
+
+
 def run_copy(args, gpu):
     """Counter calibration: one device-to-device copy of exactly 256 MiB
     (HIP runtime blit).  Under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE its
@@ -377,7 +476,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["single", "batched", "serve", "large", "copy"],
+    ap.add_argument("--workload",
+                    choices=["single", "batched", "serve", "large", "bh", "copy"],
                     default="single")
     ap.add_argument("--rates", default="20,50,80", help="serve: request rates (1/s)")
     ap.add_argument("--requests", type=int, default=150, help="serve: requests per rate")
@@ -392,12 +492,12 @@ def main():
         sys.exit(2)
     if args.steps is None:
         args.steps = {"single": 200, "batched": 50, "serve": 0, "large": 2,
-                      "copy": 1}[args.workload]
+                      "bh": 5, "copy": 1}[args.workload]
     gpu = nb.GpuContext(0)
     name, cus = gpu.device_info()
     log(f"device {name}, {cus} CUs")
     rec = {"single": run_single, "batched": run_batched, "serve": run_serve,
-           "large": run_large, "copy": run_copy}[args.workload](args, gpu)
+           "large": run_large, "bh": run_bh, "copy": run_copy}[args.workload](args, gpu)
     rec["device"] = {"arch": name, "compute_units": cus}
     print(json.dumps(rec), flush=True)
 

@@ -72,6 +72,9 @@ def _load():
     lib.oracle_integrate_record.restype = ctypes.c_int
     lib.oracle_barnes_hut.argtypes = [i64, d, d, d, _D, _D, _D, _D, _D, _D, _D, i]
     lib.oracle_barnes_hut.restype = ctypes.c_int
+    lib.oracle_barnes_hut_targets.argtypes = [i64, d, d, d, _D, _D, _D, _D, _I64,
+                                              i64, _D, _D, _D, i]
+    lib.oracle_barnes_hut_targets.restype = ctypes.c_int
     lib.oracle_bh_morton.argtypes = [ctypes.c_uint64] * 3
     lib.oracle_bh_morton.restype = ctypes.c_uint64
     _I32 = ctypes.POINTER(ctypes.c_int32)
@@ -205,6 +208,23 @@ def barnes_hut(m, x, y, z, g: float, softening: float, theta: float,
                                _p(ax), _p(ay), _p(az), nthreads or default_threads())
     if rc != 0:
         raise ValueError("oracle_barnes_hut failed")
+    return ax, ay, az
+
+
+def barnes_hut_targets(m, x, y, z, g: float, softening: float, theta: float, idx,
+                       nthreads: int | None = None):
+    """barnes_hut() for the targets idx only (the full tree is built)."""
+    lib = _load()
+    m, x, y, z = (_f64(a) for a in (m, x, y, z))
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    k = idx.shape[0]
+    ax, ay, az = np.zeros(k), np.zeros(k), np.zeros(k)
+    rc = lib.oracle_barnes_hut_targets(m.shape[0], g, softening, theta, _p(m), _p(x),
+                                       _p(y), _p(z), idx.ctypes.data_as(_I64), k,
+                                       _p(ax), _p(ay), _p(az),
+                                       nthreads or default_threads())
+    if rc != 0:
+        raise ValueError("oracle_barnes_hut_targets failed")
     return ax, ay, az
 
 

@@ -337,6 +337,52 @@ static void *bh_worker(void *p) {
   return NULL;
 }
 
+typedef struct {
+  const bh_tree *t;
+  double g, eps2, theta;
+  const int64_t *idx;
+  int64_t lo, hi;
+  double *ax, *ay, *az;
+} bh_tjob;
+
+static void *bh_tworker(void *p) {
+  bh_tjob *j = (bh_tjob *)p;
+  for (int64_t q = j->lo; q < j->hi; ++q)
+    bh_accel_for_body(j->t, j->idx[q], j->g, j->eps2, j->theta, &j->ax[q],
+                      &j->ay[q], &j->az[q]);
+  return NULL;
+}
+
+/* The same evaluation for the k targets idx[0..k) only (one full tree
+ * build, then those targets): a sampled check of large systems.  Output q
+ * is target idx[q].  Returns 0 or -1. */
+int oracle_barnes_hut_targets(int64_t n, double g, double softening,
+                              double theta, const double *m,
+                              const double *rx, const double *ry,
+                              const double *rz, const int64_t *idx,
+                              int64_t k, double *ax, double *ay, double *az,
+                              int nthreads) {
+  if (n <= 0 || k < 0) return -1;
+  for (int64_t q = 0; q < k; ++q)
+    if (idx[q] < 0 || idx[q] >= n) return -1;
+  bh_tree t;
+  if (tree_build(&t, n, m, rx, ry, rz) != 0) return -1;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  if (k > 0 && nthreads > k) nthreads = (int)k;
+  pthread_t th[256];
+  bh_tjob jobs[256];
+  for (int w = 0; w < nthreads; ++w)
+    jobs[w] = (bh_tjob){&t, g, softening * softening, theta, idx,
+                        k * w / nthreads, k * (w + 1) / nthreads, ax, ay, az};
+  for (int w = 1; w < nthreads; ++w)
+    pthread_create(&th[w], NULL, bh_tworker, &jobs[w]);
+  bh_tworker(&jobs[0]);
+  for (int w = 1; w < nthreads; ++w) pthread_join(th[w], NULL);
+  tree_free(&t);
+  return 0;
+}
+
 /* barnes_hut.rs:25-51 BarnesHutGravity::calculate_accelerations: build,
  * then every target (rayon over i; per-target arithmetic is independent of
  * the split, so any thread count gives the same bits).  Returns 0 or -1. */

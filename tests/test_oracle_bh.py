@@ -221,6 +221,15 @@ def test_theta_error_bounds():
     assert meds == sorted(meds) and meds[0] > 1e-6
 
 
+def test_sampled_targets_equal_full_evaluation():
+    m, x, y, z = plummer(1500, 8)
+    full = oracle.barnes_hut(m, x, y, z, 1.0, 0.01, 0.5)
+    idx = np.array([0, 7, 1499, 700, 3, 3])
+    part = oracle.barnes_hut_targets(m, x, y, z, 1.0, 0.01, 0.5, idx)
+    for p, q in zip(full, part):
+        assert np.array_equal(p[idx], q)
+
+
 def test_threads_do_not_change_bits():
     m, x, y, z = plummer(900, 5)
     a1 = oracle.barnes_hut(m, x, y, z, 1.0, 0.01, 0.5, nthreads=1)
