@@ -19,11 +19,15 @@
 //           reference's order (bodies of a leaf, then children in octant
 //           order)
 // Walk: one wavefront per 64 consecutive targets in Morton order, a shared
-// stack of (node, lane mask) entries in LDS.  Each lane applies the
-// reference's per-target rules; the wave opens a node's children for the
+// stack of (node, lane mask) entries in LDS.  Strict walk: each lane applies
+// the reference's per-target rules; the wave opens a node's children for the
 // lanes that need them, pushing them in octant order and popping the last
-// first.  Every lane therefore sees its own nodes in the reference's order.
-// The integrator stage epilogue runs at the end of the walk.
+// first, so every lane sees its own nodes in the reference's order.  Fast
+// walk: the stack holds nodes to open; the children of a popped node are
+// read together and judged at once (far nodes and one-body leaves add their
+// term straight away), and only children some lane must open are pushed --
+// the same set of terms, summed in another order.  The integrator stage
+// epilogue runs at the end of either walk.
 #pragma once
 #include <hip/hip_runtime_api.h>
 
@@ -33,13 +37,30 @@
 
 namespace nbody {
 
-// Device arrays of one tree (sorted bodies and preorder nodes).
+// One octree node, 128 bytes: the fields every visit reads in the first 64
+// (one scalar load), the child slots (read when the node is opened) after.
+struct alignas(128) BhNode {
+  double cx, cy, cz;      // centre of mass
+  double gm;              // g * mass, the product the pair term takes
+  double w2;              // width * width, as the opening test squares it
+  int32_t start, count;   // bodies [start, start + count) in Morton order
+  int32_t leaf, pad0;
+  double mass;            // total mass (summed into the parent)
+  int32_t child[8];       // preorder ids by octant, -1 where empty
+  int32_t pad1[8];
+};
+static_assert(sizeof(BhNode) == 128, "node record layout");
+
+// One source body in Morton order: position and g * m (32 bytes).
+struct BhBody {
+  double x, y, z, gm;
+};
+
+// Device arrays of one tree.
 struct BhTree {
-  const double *bx, *by, *bz, *bm, *bg;  // sorted positions, mass, g*m
-  const int32_t* perm;                    // sorted position -> body index
-  const double *cx, *cy, *cz, *mass, *width;
-  const int32_t *start, *count, *child;   // child: 8 slots per node, -1 empty
-  const int8_t* leaf;
+  const BhBody* body;      // sorted sources
+  const int32_t* perm;     // sorted position -> body index
+  const BhNode* node;      // preorder nodes
 };
 
 class BarnesHut {
@@ -69,14 +90,11 @@ class BarnesHut {
   int32_t* h_last_ = nullptr;  // pinned: last base and count of the scan
   uint64_t *keys_in_ = nullptr, *keys_ = nullptr;
   int32_t *idx_in_ = nullptr, *perm_ = nullptr;
-  double *bx_ = nullptr, *by_ = nullptr, *bz_ = nullptr, *bm_ = nullptr,
-         *bg_ = nullptr;
+  BhBody* body_ = nullptr;
+  double* bm_ = nullptr;  // sorted masses (leaf centres of mass)
   int32_t *cnt_ = nullptr, *base_ = nullptr;
   int8_t *top_ = nullptr, *bot_ = nullptr;
-  double *cx_ = nullptr, *cy_ = nullptr, *cz_ = nullptr, *mass_ = nullptr,
-         *width_ = nullptr;
-  int32_t *start_ = nullptr, *count_ = nullptr, *child_ = nullptr;
-  int8_t* leaf_ = nullptr;
+  BhNode* node_ = nullptr;
   double* bbox_part_ = nullptr;  // 6 per block of the bounding-box pass
   double* bbox_ = nullptr;       // min x, min y, min z, root width
   void* temp_ = nullptr;

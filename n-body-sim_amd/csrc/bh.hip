@@ -16,8 +16,6 @@ constexpr int kStack = 8 * (kDepth + 2);  // > 1 + 7*21 + 8 entries ever live
 constexpr int kThreads = 256;
 constexpr int kBboxBlocks = 256;
 
-typedef __attribute__((address_space(4))) const int8_t ci8;
-
 // ---- bounding box (octree.rs:434-468) ----------------------------------------
 __global__ __launch_bounds__(kThreads) void k_bh_bbox_part(const Arena* Ap,
                                                            int64_t off, int n,
@@ -54,14 +52,21 @@ __global__ __launch_bounds__(64) void k_bh_bbox_final(const double* part,
                                                       int nparts,
                                                       double* bbox) {
 #pragma clang fp contract(off)
-  if (threadIdx.x != 0) return;
-  double v[6];
-  for (int c = 0; c < 6; ++c) v[c] = part[c];
-  for (int p = 1; p < nparts; ++p)
+  __shared__ double s[6][64];
+  double v[6] = {HUGE_VAL, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (int p = threadIdx.x; p < nparts; p += 64)
     for (int c = 0; c < 3; ++c) {
       const double a = part[p * 6 + c], b = part[p * 6 + 3 + c];
       v[c] = a < v[c] ? a : v[c];
       v[3 + c] = b > v[3 + c] ? b : v[3 + c];
+    }
+  for (int c = 0; c < 6; ++c) s[c][threadIdx.x] = v[c];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int t = 1; t < 64; ++t)
+    for (int c = 0; c < 3; ++c) {
+      v[c] = s[c][t] < v[c] ? s[c][t] : v[c];
+      v[3 + c] = s[3 + c][t] > v[3 + c] ? s[3 + c][t] : v[3 + c];
     }
   const double dx = v[3] - v[0], dy = v[4] - v[1], dz = v[5] - v[2];
   double w = dx < dy ? dy : dx;  // f64::max on finite values
@@ -109,18 +114,18 @@ __global__ __launch_bounds__(kThreads) void k_bh_codes(const Arena* Ap,
 }
 
 // Sorted copies of the sources the walk reads.
-__global__ __launch_bounds__(kThreads) void k_bh_gather(
-    const Arena* Ap, int64_t off, int n, int src, const int32_t* perm,
-    double* bx, double* by, double* bz, double* bm, double* bg) {
+__global__ __launch_bounds__(kThreads) void k_bh_gather(const Arena* Ap,
+                                                        int64_t off, int n,
+                                                        int src,
+                                                        const int32_t* perm,
+                                                        BhBody* body,
+                                                        double* bm) {
   const int k = blockIdx.x * kThreads + threadIdx.x;
   if (k >= n) return;
   const CArena* A = (const CArena*)Ap;
   const int64_t i = off + perm[k];
-  bx[k] = A->sx[src][i];
-  by[k] = A->sy[src][i];
-  bz[k] = A->sz[src][i];
+  body[k] = BhBody{A->sx[src][i], A->sy[src][i], A->sz[src][i], A->gm64[i]};
   bm[k] = A->m[i];
-  bg[k] = A->gm64[i];
 }
 
 // ---- node chains ------------------------------------------------------------
@@ -190,22 +195,23 @@ __device__ __forceinline__ int run_start(const uint64_t* keys, int k, int b) {
 
 __global__ __launch_bounds__(kThreads) void k_bh_nodes(
     const uint64_t* keys, int n, const int32_t* base, const int8_t* top,
-    const int8_t* bot, const double* bbox, int32_t* start, int32_t* count,
-    double* width, int8_t* leaf, int32_t* child) {
+    const int8_t* bot, const double* bbox, BhNode* node) {
+#pragma clang fp contract(off)
   const int k = blockIdx.x * kThreads + threadIdx.x;
   if (k >= n) return;
   const int t = top[k], lo = bot[k];
   if (t < 0) return;
   const double root = bbox[3];
   for (int b = t; b >= lo; --b) {
-    const int id = base[k] + (t - b);
-    start[id] = k;
-    count[id] = (b == lo && (k == n - 1 || keys[k + 1] != keys[k]))
-                    ? 1
-                    : run_end(keys, n, k, b) - k;
-    width[id] = ldexp(root, b - kDepth);  // root width halved per level
-    leaf[id] = (int8_t)(b == lo);
-    for (int c = 0; c < 8; ++c) child[(int64_t)id * 8 + c] = -1;
+    BhNode& nd = node[base[k] + (t - b)];
+    nd.start = k;
+    nd.count = (b == lo && (k == n - 1 || keys[k + 1] != keys[k]))
+                   ? 1
+                   : run_end(keys, n, k, b) - k;
+    const double w = ldexp(root, b - kDepth);  // root width halved per level
+    nd.w2 = w * w;                             // octree.rs:227-228
+    nd.leaf = b == lo;
+    for (int c = 0; c < 8; ++c) nd.child[c] = -1;
   }
 }
 
@@ -215,7 +221,7 @@ __global__ __launch_bounds__(kThreads) void k_bh_link(const uint64_t* keys,
                                                       const int32_t* base,
                                                       const int8_t* top,
                                                       const int8_t* bot,
-                                                      int32_t* child) {
+                                                      BhNode* node) {
   const int k = blockIdx.x * kThreads + threadIdx.x;
   if (k >= n) return;
   const int t = top[k], lo = bot[k];
@@ -230,42 +236,41 @@ __global__ __launch_bounds__(kThreads) void k_bh_link(const uint64_t* keys,
       const int s = run_start(keys, k, b + 1);
       parent = base[s] + (top[s] - (b + 1));
     }
-    child[(int64_t)parent * 8 + (int)((c >> (3 * b)) & 7)] = id;
+    node[parent].child[(int)((c >> (3 * b)) & 7)] = id;
   }
 }
 
 // Centres of mass of the nodes of level L (octree.rs:381-432): a leaf sums
 // its bodies in Morton order, an internal node its children in octant
-// order; each total divides only when the mass is positive.
+// order; each total divides only when the mass is positive.  Also g * mass,
+// the product the pair term forms (newton.rs:139).
 __global__ __launch_bounds__(kThreads) void k_bh_mass(
-    int level, int n, const int32_t* base, const int8_t* top,
-    const int8_t* bot, const int32_t* start, const int32_t* count,
-    const int8_t* leaf, const int32_t* child, const double* bm,
-    const double* bx, const double* by, const double* bz, double* mass,
-    double* cx, double* cy, double* cz) {
+    int level, int n, double g, const int32_t* base, const int8_t* top,
+    const int8_t* bot, const BhBody* body, const double* bm, BhNode* node) {
 #pragma clang fp contract(off)
   const int k = blockIdx.x * kThreads + threadIdx.x;
   if (k >= n) return;
   const int t = top[k];
   if (t < 0 || level > t || level < bot[k]) return;
-  const int id = base[k] + (t - level);
+  BhNode& nd = node[base[k] + (t - level)];
   double tm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-  if (leaf[id]) {
-    const int s = start[id], e = s + count[id];
+  if (nd.leaf) {
+    const int s = nd.start, e = s + nd.count;
     for (int b = s; b < e; ++b) {
       tm = tm + bm[b];
-      sx = sx + bm[b] * bx[b];
-      sy = sy + bm[b] * by[b];
-      sz = sz + bm[b] * bz[b];
+      sx = sx + bm[b] * body[b].x;
+      sy = sy + bm[b] * body[b].y;
+      sz = sz + bm[b] * body[b].z;
     }
   } else {
     for (int c = 0; c < 8; ++c) {
-      const int ch = child[(int64_t)id * 8 + c];
+      const int ch = nd.child[c];
       if (ch < 0) continue;
-      tm = tm + mass[ch];
-      sx = sx + mass[ch] * cx[ch];
-      sy = sy + mass[ch] * cy[ch];
-      sz = sz + mass[ch] * cz[ch];
+      const BhNode& cn = node[ch];
+      tm = tm + cn.mass;
+      sx = sx + cn.mass * cn.cx;
+      sy = sy + cn.mass * cn.cy;
+      sz = sz + cn.mass * cn.cz;
     }
   }
   if (tm > 0.0) {
@@ -273,10 +278,11 @@ __global__ __launch_bounds__(kThreads) void k_bh_mass(
     sy = sy / tm;
     sz = sz / tm;
   }
-  mass[id] = tm;
-  cx[id] = sx;
-  cy[id] = sy;
-  cz[id] = sz;
+  nd.mass = tm;
+  nd.gm = g * tm;
+  nd.cx = sx;
+  nd.cy = sy;
+  nd.cz = sz;
 }
 
 // ---- walk (octree.rs:159-254) --------------------------------------------------
@@ -308,12 +314,23 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-template <bool STRICT>
-__global__ __launch_bounds__(kWave) void k_bh_walk(const Arena* Ap,
-                                                   const SysDesc* sysd,
-                                                   int sys, int stage,
-                                                   int src, BhTree T,
-                                                   double theta2, int n) {
+// The fields of a node record every visit reads (the first 52 bytes).
+struct NodeHot {
+  double cx, cy, cz, gm, w2;
+  int32_t start, count, leaf;
+};
+struct NodeKids {
+  int32_t c[8];
+};
+
+// Strict walk: the reference's traversal per lane (octree.rs:159-254),
+// its arithmetic and its order.
+__global__ __launch_bounds__(kWave) void k_bh_walk_strict(const Arena* Ap,
+                                                          const SysDesc* sysd,
+                                                          int sys, int stage,
+                                                          int src, BhTree T,
+                                                          double theta2,
+                                                          int n) {
 #pragma clang fp contract(off)
   __shared__ int32_t st_node[kStack];
   __shared__ uint64_t st_mask[kStack];
@@ -323,16 +340,9 @@ __global__ __launch_bounds__(kWave) void k_bh_walk(const Arena* Ap,
   const int kk = valid ? k : n - 1;
   const CArena* A = (const CArena*)Ap;
   const CSys* sd = (const CSys*)(sysd + sys);
-  const double eps2 = sd->eps2, g = sd->g;
-  const double tx = T.bx[kk], ty = T.by[kk], tz = T.bz[kk];
-  const cf64 *CX = (const cf64*)T.cx, *CY = (const cf64*)T.cy,
-             *CZ = (const cf64*)T.cz, *M = (const cf64*)T.mass,
-             *W = (const cf64*)T.width, *BX = (const cf64*)T.bx,
-             *BY = (const cf64*)T.by, *BZ = (const cf64*)T.bz,
-             *BG = (const cf64*)T.bg;
-  const ci32 *START = (const ci32*)T.start, *COUNT = (const ci32*)T.count,
-             *CHILD = (const ci32*)T.child;
-  const ci8* LEAF = (const ci8*)T.leaf;
+  const double eps2 = sd->eps2;
+  const BhBody me = T.body[kk];
+  const double tx = me.x, ty = me.y, tz = me.z;
   double ax = 0.0, ay = 0.0, az = 0.0;
   const uint64_t live = __ballot(valid);
   if (lane == 0) {  // the root, for every live lane
@@ -343,38 +353,39 @@ __global__ __launch_bounds__(kWave) void k_bh_walk(const Arena* Ap,
   int sp = 1;
   while (sp > 0) {
     --sp;
-    const int node = __builtin_amdgcn_readfirstlane(st_node[sp]);
+    const int id = __builtin_amdgcn_readfirstlane(st_node[sp]);
     const uint64_t mask = uniform64(st_mask[sp]);
     __syncthreads();
+    const NodeHot nd = load_uniform((const NodeHot*)(T.node + id));
     const bool act = (mask >> lane) & 1;
-    const double ndx = CX[node] - tx, ndy = CY[node] - ty, ndz = CZ[node] - tz;
+    const double ndx = nd.cx - tx, ndy = nd.cy - ty, ndz = nd.cz - tz;
     const double d2 = ((ndx * ndx) + (ndy * ndy)) + (ndz * ndz);
     const bool here = act && d2 != 0.0;  // a node at the target is skipped
-    if (LEAF[node]) {
+    if (nd.leaf) {
       if (__ballot(here)) {
-        const int s = START[node], e = s + COUNT[node];
-        for (int b = s; b < e; ++b)
-          add_pair<STRICT>(BG[b], BX[b] - tx, BY[b] - ty, BZ[b] - tz, eps2,
+        for (int b = nd.start; b < nd.start + nd.count; ++b) {
+          const BhBody s = load_uniform(T.body + b);
+          add_pair<true>(s.gm, s.x - tx, s.y - ty, s.z - tz, eps2,
                            here && b != k, ax, ay, az);
+        }
       }
       continue;
     }
-    const double w = W[node];
+    const bool far = (nd.w2 / d2) < theta2;  // octree.rs:227-229
     bool near = false;
     if (here) {
-      if (((w * w) / d2) < theta2) {  // far enough: one mass at the centre
-        add_pair<STRICT>(g * M[node], ndx, ndy, ndz, eps2, true, ax, ay, az);
-      } else {
+      if (far)  // far enough: one mass at the centre
+        add_pair<true>(nd.gm, ndx, ndy, ndz, eps2, true, ax, ay, az);
+      else
         near = true;
-      }
     }
     const uint64_t open = __ballot(near);
     if (open) {
+      const NodeKids kids = load_uniform((const NodeKids*)T.node[id].child);
       for (int c = 0; c < 8; ++c) {
-        const int ch = CHILD[(int64_t)node * 8 + c];
-        if (ch >= 0) {
+        if (kids.c[c] >= 0) {
           if (lane == 0) {
-            st_node[sp] = ch;
+            st_node[sp] = kids.c[c];
             st_mask[sp] = open;
           }
           ++sp;
@@ -385,8 +396,109 @@ __global__ __launch_bounds__(kWave) void k_bh_walk(const Arena* Ap,
   }
   if (valid) {
     const int64_t i = sd->off + T.perm[k];
-    epilogue<STRICT ? MODE_F64_STRICT : MODE_F64>(A, sd, stage, src ^ 1, i, ax,
-                                                  ay, az);
+    epilogue<MODE_F64_STRICT>(A, sd, stage, src ^ 1, i, ax, ay, az);
+  }
+}
+
+// Fast walk (f64, fused multiply-add, rsq-based inverse square root).
+// Judges one node for every lane: d^2 == 0 skips; a one-body leaf adds its
+// body (through the node's centre of mass and g*m, which equal the body's to
+// rounding) unless it is the lane's own; a larger leaf adds its bodies; an
+// internal node adds its mass when w^2 < theta^2 d^2 and otherwise joins the
+// returned open mask.
+__device__ __forceinline__ uint64_t judge_fast(const NodeHot& r, const BhTree& T,
+                                               int k, double tx, double ty,
+                                               double tz, double eps2,
+                                               double theta2, bool act,
+                                               double& ax, double& ay,
+                                               double& az) {
+  const double ndx = r.cx - tx, ndy = r.cy - ty, ndz = r.cz - tz;
+  const double d2 = fma(ndz, ndz, fma(ndy, ndy, ndx * ndx));
+  const bool here = act && d2 != 0.0;
+  if (r.leaf) {
+    if (r.count == 1) {
+      add_pair<false>(r.gm, ndx, ndy, ndz, eps2, here && r.start != k, ax, ay,
+                      az);
+    } else if (__ballot(here)) {
+      for (int b = r.start; b < r.start + r.count; ++b) {
+        const BhBody s = load_uniform(T.body + b);
+        add_pair<false>(s.gm, s.x - tx, s.y - ty, s.z - tz, eps2,
+                        here && b != k, ax, ay, az);
+      }
+    }
+    return 0;
+  }
+  const bool far = r.w2 < theta2 * d2;
+  add_pair<false>(r.gm, ndx, ndy, ndz, eps2, here && far, ax, ay, az);
+  return __ballot(here && !far);
+}
+
+__global__ __launch_bounds__(kWave) void k_bh_walk_fast(const Arena* Ap,
+                                                        const SysDesc* sysd,
+                                                        int sys, int stage,
+                                                        int src, BhTree T,
+                                                        double theta2, int n) {
+  __shared__ int32_t st_node[kStack];
+  __shared__ uint64_t st_mask[kStack];
+  const int lane = (int)threadIdx.x;
+  const int k = (int)blockIdx.x * kWave + lane;
+  const bool valid = k < n;
+  const int kk = valid ? k : n - 1;
+  const CArena* A = (const CArena*)Ap;
+  const CSys* sd = (const CSys*)(sysd + sys);
+  const double eps2 = sd->eps2;
+  const BhBody me = T.body[kk];
+  const double tx = me.x, ty = me.y, tz = me.z;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  int sp = 0;
+  {
+    const NodeHot root = load_uniform((const NodeHot*)T.node);
+    const uint64_t open = judge_fast(root, T, k, tx, ty, tz, eps2, theta2,
+                                     valid, ax, ay, az);
+    if (open) {
+      if (lane == 0) {
+        st_node[0] = 0;
+        st_mask[0] = open;
+      }
+      sp = 1;
+    }
+  }
+  __syncthreads();
+  while (sp > 0) {
+    --sp;
+    const int id = __builtin_amdgcn_readfirstlane(st_node[sp]);
+    const uint64_t mask = uniform64(st_mask[sp]);
+    __syncthreads();
+    const bool act = (mask >> lane) & 1;
+    const NodeKids kids = load_uniform((const NodeKids*)T.node[id].child);
+    // four children per batch: their records load together (an empty slot
+    // reads the root's record and is then skipped)
+#pragma unroll
+    for (int c0 = 0; c0 < 8; c0 += 4) {
+      NodeHot r[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        r[q] = load_uniform((const NodeHot*)(T.node + max(kids.c[c0 + q], 0)));
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int ch = kids.c[c0 + q];
+        if (ch < 0) continue;
+        const uint64_t open = judge_fast(r[q], T, k, tx, ty, tz, eps2, theta2,
+                                         act, ax, ay, az);
+        if (open) {
+          if (lane == 0) {
+            st_node[sp] = ch;
+            st_mask[sp] = open;
+          }
+          ++sp;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (valid) {
+    const int64_t i = sd->off + T.perm[k];
+    epilogue<MODE_F64>(A, sd, stage, src ^ 1, i, ax, ay, az);
   }
 }
 
@@ -417,7 +529,7 @@ BarnesHut::BarnesHut(int64_t n) : n_(n) {
   };
   const size_t o_keys_in = take(8 * n), o_keys = take(8 * n);
   const size_t o_idx_in = take(4 * n), o_perm = take(4 * n);
-  const size_t o_b = take(8 * 5 * n);
+  const size_t o_body = take(sizeof(BhBody) * n), o_bm = take(8 * n);
   const size_t o_cnt = take(4 * n), o_base = take(4 * n);
   const size_t o_top = take(n), o_bot = take(n);
   const size_t o_bbox = take(8 * (6 * kBboxBlocks + 4));
@@ -428,11 +540,8 @@ BarnesHut::BarnesHut(int64_t n) : n_(n) {
   keys_ = (uint64_t*)(p + o_keys);
   idx_in_ = (int32_t*)(p + o_idx_in);
   perm_ = (int32_t*)(p + o_perm);
-  bx_ = (double*)(p + o_b);
-  by_ = bx_ + n;
-  bz_ = by_ + n;
-  bm_ = bz_ + n;
-  bg_ = bm_ + n;
+  body_ = (BhBody*)(p + o_body);
+  bm_ = (double*)(p + o_bm);
   cnt_ = (int32_t*)(p + o_cnt);
   base_ = (int32_t*)(p + o_base);
   top_ = (int8_t*)(p + o_top);
@@ -452,31 +561,16 @@ BarnesHut::~BarnesHut() {
   (void)hipHostFree(h_last_);
 }
 
-// Node arrays for `cap` nodes (grow only; the previous ones are unused
+// Node records for `cap` nodes (grow only; the previous ones are unused
 // once the stream has drained, which the callers' synchronisation ensures).
 void BarnesHut::reserve_nodes(int64_t cap) {
   if (cap <= cap_) return;
   (void)hipFree(nodes_);
   nodes_ = nullptr;
   cap_ = cap;
-  const size_t per = 8 * 5 + 4 + 4 + 4 * 8 + 1;  // bytes per node
-  hip_check(hipMalloc(&nodes_, per * (size_t)cap_ + 5 * 256),
+  hip_check(hipMalloc(&nodes_, sizeof(BhNode) * (size_t)cap_),
             "hipMalloc(barnes-hut nodes)");
-  char* p = (char*)nodes_;
-  auto next = [&](size_t bytes) {
-    char* at = p;
-    p += (bytes + 255) & ~(size_t)255;
-    return at;
-  };
-  cx_ = (double*)next(8 * 5 * (size_t)cap_);
-  cy_ = cx_ + cap_;
-  cz_ = cy_ + cap_;
-  mass_ = cz_ + cap_;
-  width_ = mass_ + cap_;
-  start_ = (int32_t*)next(4 * (size_t)cap_);
-  count_ = (int32_t*)next(4 * (size_t)cap_);
-  child_ = (int32_t*)next(4 * 8 * (size_t)cap_);
-  leaf_ = (int8_t*)next((size_t)cap_);
+  node_ = (BhNode*)nodes_;
 }
 
 void BarnesHut::evaluate(hipStream_t s, bool strict, const Arena* d_arena,
@@ -497,7 +591,7 @@ void BarnesHut::evaluate(hipStream_t s, bool strict, const Arena* d_arena,
                                                idx_in_, perm_, n, 0, 63, s),
             "radix sort");
   hipLaunchKernelGGL(k_bh_gather, dim3(blocks(n, kThreads)), dim3(kThreads), 0,
-                     s, d_arena, off, n, src, perm_, bx_, by_, bz_, bm_, bg_);
+                     s, d_arena, off, n, src, perm_, body_, bm_);
   hipLaunchKernelGGL(k_bh_chain, dim3(blocks(n, kThreads)), dim3(kThreads), 0,
                      s, keys_, n, cnt_, top_, bot_);
   bytes = temp_bytes_;
@@ -509,23 +603,21 @@ void BarnesHut::evaluate(hipStream_t s, bool strict, const Arena* d_arena,
     throw Error(NBODY_E_DEVICE, "barnes-hut node count out of range");
   reserve_nodes(total);
   hipLaunchKernelGGL(k_bh_nodes, dim3(blocks(n, kThreads)), dim3(kThreads), 0,
-                     s, keys_, n, base_, top_, bot_, bbox_, start_, count_,
-                     width_, leaf_, child_);
+                     s, keys_, n, base_, top_, bot_, bbox_, node_);
   hipLaunchKernelGGL(k_bh_link, dim3(blocks(n, kThreads)), dim3(kThreads), 0,
-                     s, keys_, n, base_, top_, bot_, child_);
+                     s, keys_, n, base_, top_, bot_, node_);
   for (int level = 0; level <= kDepth; ++level)
     hipLaunchKernelGGL(k_bh_mass, dim3(blocks(n, kThreads)), dim3(kThreads), 0,
-                       s, level, n, base_, top_, bot_, start_, count_, leaf_,
-                       child_, bm_, bx_, by_, bz_, mass_, cx_, cy_, cz_);
-  BhTree T{bx_, by_, bz_, bm_, bg_, perm_, cx_, cy_, cz_, mass_, width_,
-           start_, count_, child_, leaf_};
+                       s, level, n, desc.g, base_, top_, bot_, body_, bm_,
+                       node_);
+  BhTree T{body_, perm_, node_};
   const double theta2 = theta * theta;  // octree.rs:229
   if (strict)
-    hipLaunchKernelGGL(k_bh_walk<true>, dim3(blocks(n, kWave)), dim3(kWave), 0,
+    hipLaunchKernelGGL(k_bh_walk_strict, dim3(blocks(n, kWave)), dim3(kWave), 0,
                        s, d_arena, d_sys, sys, stage, src, T, theta2, n);
   else
-    hipLaunchKernelGGL(k_bh_walk<false>, dim3(blocks(n, kWave)), dim3(kWave),
-                       0, s, d_arena, d_sys, sys, stage, src, T, theta2, n);
+    hipLaunchKernelGGL(k_bh_walk_fast, dim3(blocks(n, kWave)), dim3(kWave), 0,
+                       s, d_arena, d_sys, sys, stage, src, T, theta2, n);
   hip_check(hipGetLastError(), "barnes-hut launch");
 }
 
