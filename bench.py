@@ -222,11 +222,13 @@ def run_serve(args, gpu):
     rates = [float(r) for r in args.rates.split(",")]
     nreq = args.requests
     # pre-generate the request stream (host side, not timed)
+    kinds = args.mix.split(",")
+    make = {"plummer": nb.plummer, "star": nb.star_system, "disc": nb.disc_galaxy}
     reqs = []
     for k in range(nreq):
         n = int(np.exp(rng.uniform(np.log(32), np.log(16384))))
-        b = nb.plummer(n, 50_000 + k) if rng.random() < 0.5 else \
-            nb.star_system(n, 50_000 + k)
+        kind = kinds[int(rng.random() * len(kinds))]
+        b = make[kind](n, 50_000 + k)
         reqs.append(dict(b=b, integ=nb.IntegratorMethod(int(rng.integers(0, 3))),
                          dt=float(rng.choice([1e-3, 5e-4])),
                          steps=int(rng.integers(10, 1001))))
@@ -280,7 +282,7 @@ def run_serve(args, gpu):
         "steps": None, "warmup": 0, "ms_per_step": None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 pair terms (f64 differences), f64 state",
-        "data": "synthetic Poisson request stream (Plummer / star systems)",
+        "data": f"synthetic Poisson request stream ({args.mix.replace(',', ' / ')})",
         "config": {"workload": "served requests, N log-uniform [32, 16384], steps "
                                "uniform [10, 1000], random integrator, dt in "
                                "{1e-3, 5e-4}, snapshots every 10 steps",
@@ -481,6 +483,8 @@ def main():
                     default="single")
     ap.add_argument("--rates", default="20,50,80", help="serve: request rates (1/s)")
     ap.add_argument("--requests", type=int, default=150, help="serve: requests per rate")
+    ap.add_argument("--mix", default="plummer,star",
+                    help="serve: request scenarios, from plummer, star, disc")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()

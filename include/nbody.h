@@ -190,6 +190,16 @@ int nbody_scenario_colliding(int64_t n, uint64_t seed, double separation,
                              double rel_speed, double g, double *masses,
                              double *pos_x, double *pos_y, double *pos_z,
                              double *vel_x, double *vel_y, double *vel_z);
+/* Disc galaxy after the GUI's generator (gui/generators.py:101-153): n
+ * bodies of body_mass, radii radius * U^density_power, a thin layer, each on
+ * the softened circular speed of the mass inside its radius plus 10 %
+ * dispersion; written in order of radius.  The GUI's defaults are radius 20,
+ * body_mass 1e-3, density_power 1.5, softening 0.5. */
+int nbody_scenario_disc(int64_t n, uint64_t seed, double radius,
+                        double body_mass, double density_power,
+                        double softening, double g, double *masses,
+                        double *pos_x, double *pos_y, double *pos_z,
+                        double *vel_x, double *vel_y, double *vel_z);
 
 /* ---- batched form: many independent systems of uneven N in one arena,
  * stepped by shared launches captured in a HIP graph.  Bodies of system s

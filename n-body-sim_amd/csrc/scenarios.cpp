@@ -1,7 +1,9 @@
 // Seeded scenario generators (host only).  Portable and deterministic: own
 // xoshiro256** stream seeded through splitmix64, own Box-Muller, no
 // implementation-defined std::*_distribution.
+#include <algorithm>
 #include <cmath>
+#include <vector>
 #include <cstdint>
 
 #include "../../include/nbody.h"
@@ -33,6 +35,11 @@ struct Rng {  // xoshiro256** (Blackman & Vigna 2018), splitmix64 seeding
   }
   double u01() { return (double)(next() >> 11) * 0x1.0p-53; }  // [0, 1)
   double uniform(double lo, double hi) { return lo + (hi - lo) * u01(); }
+  double gauss(double mean, double sd) {  // Box-Muller (1958), one draw pair
+    const double u1 = 1.0 - u01();         // (0, 1]
+    const double u2 = u01();
+    return mean + sd * std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+  }
   void unit_vector(double& x, double& y, double& z) {  // isotropic
     const double c = 2.0 * u01() - 1.0;
     const double ph = 2.0 * M_PI * u01();
@@ -193,6 +200,50 @@ extern "C" int nbody_scenario_colliding(int64_t n, uint64_t seed,
       x[i] += cx[s];
       vx[i] += cv[s];
     }
+  }
+  return NBODY_OK;
+}
+
+// Disc galaxy after the GUI's generate_disc_galaxy (gui/generators.py:
+// 101-153): radii radius * U^density_power, uniform azimuth, a thin
+// Gaussian layer (sd 0.01 radius); bodies written in order of radius, each
+// on the circular speed of the mass inside it, softened,
+// sqrt(g * m_enc * r / (r^2 + softening^2)), plus 10 % velocity dispersion
+// in the plane and 5 % vertically.  Draws follow the GUI's order: all
+// positions, then the velocities in radius order.
+extern "C" int nbody_scenario_disc(int64_t n, uint64_t seed, double radius,
+                                   double body_mass, double density_power,
+                                   double softening, double g, double* m,
+                                   double* x, double* y, double* z,
+                                   double* vx, double* vy, double* vz) {
+  if (n <= 0 || !(radius > 0) || !(body_mass > 0) || !(density_power > 0) ||
+      !(softening >= 0) || !(g > 0) || !std::isfinite(radius) ||
+      !ok_ptrs(m, x, y, z, vx, vy, vz))
+    return NBODY_E_INVALID;
+  Rng rng(seed);
+  std::vector<double> r(n), th(n), h(n);
+  for (int64_t i = 0; i < n; ++i) {
+    r[i] = radius * std::pow(rng.u01(), density_power);
+    th[i] = rng.uniform(0.0, 2.0 * M_PI);
+    h[i] = rng.gauss(0.0, radius * 0.01);
+  }
+  std::vector<int64_t> order(n);
+  for (int64_t i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t b) { return r[a] < r[b]; });
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t i = order[k];
+    const double menc = (double)(k + 1) * body_mass;
+    const double v = std::sqrt(g * menc * r[i] /
+                               (r[i] * r[i] + softening * softening));
+    const double disp = v * 0.1;
+    m[k] = body_mass;
+    x[k] = r[i] * std::cos(th[i]);
+    y[k] = r[i] * std::sin(th[i]);
+    z[k] = h[i];
+    vx[k] = -v * std::sin(th[i]) + rng.gauss(0.0, disp);
+    vy[k] = v * std::cos(th[i]) + rng.gauss(0.0, disp);
+    vz[k] = rng.gauss(0.0, disp * 0.5);
   }
   return NBODY_OK;
 }

@@ -76,6 +76,7 @@ def _bind(lib):
         "nbody_scenario_plummer": ([i64, u64, d, d, d] + [_D] * 7, i),
         "nbody_scenario_star_system": ([i64, u64, d, d, d, d, d, d, d] + [_D] * 7, i),
         "nbody_scenario_colliding": ([i64, u64, d, d, d] + [_D] * 7, i),
+        "nbody_scenario_disc": ([i64, u64, d, d, d, d, d] + [_D] * 7, i),
         "nbody_batch_create": ([vp, i, ctypes.POINTER(i64), ctypes.POINTER(i), _D, _D,
                                 _D, i] + [_D] * 7 + [pp], i),
         "nbody_batch_step": ([vp, vp, i64], i),
@@ -576,6 +577,14 @@ def _scenario(fn, n, *args) -> Bodies:
 def plummer(n: int, seed: int, a: float = 1.0, total_mass: float = 1.0,
             g: float = 1.0) -> Bodies:
     return _scenario(load_library().nbody_scenario_plummer, n, seed, a, total_mass, g)
+
+
+def disc_galaxy(n: int, seed: int, radius: float = 20.0, body_mass: float = 1e-3,
+                density_power: float = 1.5, softening: float = 0.5,
+                g: float = 1.0) -> Bodies:
+    """gui/generators.py:101-153 generate_disc_galaxy (own seeded RNG)."""
+    return _scenario(load_library().nbody_scenario_disc, n, seed, radius, body_mass,
+                     density_power, softening, g)
 
 
 def star_system(n: int, seed: int, central_mass: float = 1.0, m_lo: float = 1e-7,

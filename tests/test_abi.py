@@ -117,6 +117,29 @@ def test_star_system_properties():
     assert incl.max() < 5.0 + 1e-9
 
 
+def test_disc_galaxy_properties():
+    R, mb, p, soft = 20.0, 1e-3, 1.5, 0.5
+    b = nb.disc_galaxy(20000, 4, radius=R, body_mass=mb, density_power=p, softening=soft)
+    assert np.all(b.masses == mb)
+    r = np.hypot(b.pos_x, b.pos_y)
+    assert np.all(np.diff(r) >= 0) and r.max() <= R          # written by radius
+    # radii R * U^p: median R * 0.5^p
+    assert abs(np.median(r) / (R * 0.5 ** p) - 1) < 0.03
+    assert abs(b.pos_z.std() / (0.01 * R) - 1) < 0.03         # thin layer
+    # softened circular speed of the enclosed mass, 10 % dispersion
+    menc = (np.arange(20000) + 1) * mb
+    v = np.sqrt(menc * r / (r * r + soft * soft))
+    th = np.arctan2(b.pos_y, b.pos_x)
+    vt = -b.vel_x * np.sin(th) + b.vel_y * np.cos(th)
+    vr = b.vel_x * np.cos(th) + b.vel_y * np.sin(th)
+    assert abs(np.mean((vt - v) / v)) < 0.01
+    assert abs(np.std((vt - v) / v) - 0.1) < 0.005
+    assert abs(np.std(vr / v) - 0.1) < 0.005
+    assert abs(np.std(b.vel_z / v) - 0.05) < 0.003
+    a, c = nb.disc_galaxy(100, 1), nb.disc_galaxy(100, 1)
+    assert all(np.array_equal(p1, q1) for p1, q1 in zip(a.arrays(), c.arrays()))
+
+
 def test_colliding_properties():
     b = nb.colliding(2000, 5, separation=10.0, rel_speed=0.5)
     assert np.allclose(b.masses, 1.0 / 2000)
