@@ -54,21 +54,15 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
   return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
-// Writes the next source snapshot of body i: f64 positions (f64 modes and
-// MODE_F32C), or f32 positions relative to the system origin (MODE_F32R).
+// Writes the next source snapshot of body i (f64 positions, every mode).
 template <int MODE>
 __device__ __forceinline__ void put_snapshot(const CArena* A, int dst,
                                              int64_t i, const CSys* sd,
                                              double x, double y, double z) {
-  if (MODE == MODE_F32R) {
-    A->fx[dst][i] = (float)(x - sd->ox);
-    A->fy[dst][i] = (float)(y - sd->oy);
-    A->fz[dst][i] = (float)(z - sd->oz);
-  } else {
-    A->sx[dst][i] = x;
-    A->sy[dst][i] = y;
-    A->sz[dst][i] = z;
-  }
+  (void)sd;
+  A->sx[dst][i] = x;
+  A->sy[dst][i] = y;
+  A->sz[dst][i] = z;
 }
 
 // Integrator stage for body i, given its fresh acceleration (ax, ay, az).

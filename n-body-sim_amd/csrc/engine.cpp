@@ -58,6 +58,10 @@ Engine::~Engine() {
   hipFree(d_stage_);
   hipHostFree(h_stage_);
   hipFree(d_traj_);
+  hipFree(d_sort_tab_);
+  hipFree(d_keys_);
+  hipFree(d_vals_);
+  hipFree(d_sort_tmp_);
   hipFree(d_diag_);
   hipFree(d_list_);
   hipFree(d_sys_);
@@ -105,12 +109,8 @@ size_t Engine::carve(char* base) {
   A_.py = D(kMaxSplit);
   A_.pz = D(kMaxSplit);
   A_.pstride = (int64_t)(dbytes / sizeof(double));
+  A_.perm = (int32_t*)F();
   A_.gm32 = F();
-  for (int b = 0; b < 2; ++b) {
-    A_.fx[b] = F();
-    A_.fy[b] = F();
-    A_.fz[b] = F();
-  }
   A_.cap = cap_;
   return used;
 }
@@ -238,8 +238,9 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
   s.off = off;
   s.span = need;
   s.split = split_for(p.n);
-  // centre of mass and its velocity, f64 in index order (origin of the f32
-  // snapshots; advanced by vcom * elapsed time at every re-centring)
+  // centre of mass and its velocity, f64 in index order (the centre of the
+  // MODE_F32R ordering box, advanced by vcom * elapsed time), and the
+  // largest coordinate distance from it (the box size)
   double M = 0, c[3] = {0, 0, 0}, v[3] = {0, 0, 0};
   for (int64_t i = 0; i < p.n; ++i) {
     M += m[i];
@@ -254,6 +255,9 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
     s.com[q] = (M != 0.0) ? c[q] / M : 0.0;
     s.vcom[q] = (M != 0.0) ? v[q] / M : 0.0;
   }
+  for (int64_t i = 0; i < p.n; ++i)
+    s.ext = std::max({s.ext, std::fabs(x[i] - s.com[0]),
+                      std::fabs(y[i] - s.com[1]), std::fabs(z[i] - s.com[2])});
   const size_t bytes = sizeof(double) * p.n;
   double* dst[7] = {A_.m, A_.x, A_.y, A_.z, A_.vx, A_.vy, A_.vz};
   for (int q = 0; q < 7; ++q)
@@ -279,10 +283,8 @@ void Engine::remove(int id) {
   for (double* a : {A_.x, A_.y, A_.z, A_.sx[0], A_.sy[0], A_.sz[0], A_.sx[1],
                     A_.sy[1], A_.sz[1], A_.gm64})
     hip_check(launch_fill_zero(a + s.off, s.span, stream_), "fill");
-  for (float* a : {A_.fx[0], A_.fy[0], A_.fz[0], A_.fx[1], A_.fy[1],
-                   A_.fz[1], A_.gm32})
-    hip_check(hipMemsetAsync(a + s.off, 0, sizeof(float) * s.span, stream_),
-              "hipMemset");
+  hip_check(hipMemsetAsync(A_.gm32 + s.off, 0, sizeof(float) * s.span, stream_),
+            "hipMemset");
   int64_t off = s.off, len = s.span;
   auto nx = free_.lower_bound(off);
   if (nx != free_.end() && nx->first == off + len) {
@@ -332,6 +334,7 @@ void Engine::set_mode(int mode) {
   for (auto& s : sys_) {
     s.vv_primed = false;
     s.prepared = false;
+    s.sorted_at = -1;
   }
   table_dirty_ = true;  // split counts depend on the mode
   drop_graphs();
@@ -349,6 +352,7 @@ void Engine::set_gravity(int gravity, double theta) {
   for (auto& s : sys_) {
     s.vv_primed = false;
     s.prepared = false;
+    s.sorted_at = -1;
   }
   drop_graphs();
 }
@@ -377,13 +381,6 @@ int64_t Engine::tree_nodes(int id) {
   check_id(id);
   auto it = trees_.find(id);
   return it == trees_.end() ? 0 : it->second->node_count(stream_);
-}
-
-void Engine::set_origin(Sys& s, SysDesc& d) const {
-  const double t = (double)s.steps * s.p.dt;
-  d.ox = s.com[0] + s.vcom[0] * t;
-  d.oy = s.com[1] + s.vcom[1] * t;
-  d.oz = s.com[2] + s.vcom[2] * t;
 }
 
 SysDesc Engine::desc_of(int id) const {
@@ -422,11 +419,7 @@ void Engine::upload_table() {
   if (!table_dirty_) return;
   for (int k = 0; k < max_sys_; ++k) {
     if (!sys_[k].live) continue;
-    SysDesc d = desc_of(k);
-    d.ox = h_sys_[k].ox;
-    d.oy = h_sys_[k].oy;
-    d.oz = h_sys_[k].oz;
-    h_sys_[k] = d;
+    h_sys_[k] = desc_of(k);
   }
   hip_check(hipMemcpyAsync(d_sys_, h_sys_.data(), sizeof(SysDesc) * max_sys_,
                            hipMemcpyHostToDevice, stream_),
@@ -488,27 +481,13 @@ void Engine::run_round(std::vector<WorkItem>& items,
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
-// Rebuilds gm and the current snapshot of the listed systems that need it:
-// never prepared, or due for re-centring.  A system is re-centred when its
-// own step count is a multiple of its rebase interval; the new origin is the
-// centre of mass advanced to the system's current time (com + vcom * t).
-// The schedule depends only on the system itself, so a system's results do
-// not depend on which other systems share its launches.
+// Rebuilds gm and the current snapshot of the listed systems that were
+// never prepared (or whose snapshot a parameter or mode change invalidated).
 void Engine::prepare(const std::vector<int>& ids) {
   std::vector<WorkItem> items;
   for (int id : ids) {
     Sys& s = sys_[id];
-    const bool due = s.rebase_every > 0 && (s.steps % s.rebase_every) == 0;
-    if (s.prepared && !due) continue;
-    SysDesc d = desc_of(id);
-    set_origin(s, d);
-    const bool moved = d.ox != h_sys_[id].ox || d.oy != h_sys_[id].oy ||
-                       d.oz != h_sys_[id].oz;
-    if (s.prepared && !(moved && mode_ == MODE_F32R)) continue;
-    if (moved) table_dirty_ = true;
-    h_sys_[id].ox = d.ox;
-    h_sys_[id].oy = d.oy;
-    h_sys_[id].oz = d.oz;
+    if (s.prepared) continue;
     const int stage = (s.p.integ == INTEG_VV && s.vv_primed) ? STAGE_VV_PRIME
                                                              : STAGE_ACC;
     tiles_of(id, stage, s.cur, items);
@@ -525,11 +504,6 @@ void Engine::prepare(const std::vector<int>& ids) {
   hip_check(launch_prepare(d_arena_, d_items_, (int)items.size(), d_sys_, stream_),
             "prepare launch");
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-}
-
-void Engine::set_rebase_interval(int id, int64_t every) {
-  check_id(id);
-  sys_[id].rebase_every = every;
 }
 
 // Work lists of one step for the listed systems: force[r] and combine[r]
@@ -668,15 +642,22 @@ void Engine::step(const std::vector<int>& ids, int64_t nsteps) {
   for (int id : ids) check_id(id);
   if (nsteps == 0 || ids.empty()) return;
   prepare(ids);
-  prime(ids);
   if (gravity_ == GRAV_BARNES_HUT) {
+    prime(ids);
     for (int64_t t = 0; t < nsteps; ++t) bh_step(ids, t);
     finish_steps(ids, nsteps);
     return;
   }
+  const std::vector<SortPass> plan = plan_sorts(ids, nsteps);
+  size_t next = 0;
+  run_sorts(plan, next, 0);
+  prime(ids);
   StepLists L;
   upload_step_lists(ids, L);
-  for (int64_t t = 0; t < nsteps; ++t) launch_step(L, (int)(t & 1));
+  for (int64_t t = 0; t < nsteps; ++t) {
+    run_sorts(plan, next, t);
+    launch_step(L, (int)(t & 1));
+  }
   finish_steps(ids, nsteps);
 }
 
@@ -691,6 +672,9 @@ void Engine::accelerations(int id, double* ax, double* ay, double* az) {
   if (gravity_ == GRAV_BARNES_HUT) {
     bh_round(id, STAGE_ACC, s.cur);
   } else {
+    const std::vector<SortPass> plan = plan_sorts({id}, 1);
+    size_t next = 0;
+    run_sorts(plan, next, 0);
     std::vector<WorkItem> items, combine;
     tiles_of(id, STAGE_ACC, s.cur, items, &combine);
     run_round(items, combine);
@@ -848,8 +832,14 @@ void Engine::record(int id, int64_t nsteps, double* x, double* y, double* z,
     gitems.push_back(w);
   }
   prepare({id});
-  prime({id});
   const bool bh = gravity_ == GRAV_BARNES_HUT;
+  std::vector<SortPass> plan;
+  size_t next = 0;
+  if (!bh) {
+    plan = plan_sorts({id}, nsteps);
+    run_sorts(plan, next, 0);
+  }
+  prime({id});
   StepLists L;
   if (!bh) upload_step_lists({id}, L);
   validate_items(gitems);
@@ -864,10 +854,12 @@ void Engine::record(int id, int64_t nsteps, double* x, double* y, double* z,
   while (done < nsteps) {
     const int64_t c = std::min(C, nsteps - done);
     for (int64_t s = 0; s < c; ++s) {
-      if (bh)
+      if (bh) {
         bh_step({id}, done + s);
-      else
+      } else {
+        run_sorts(plan, next, done + s);
         launch_step(L, (int)((done + s) & 1));
+      }
       hip_check(launch_gather(d_arena_, d_items_, (int)gitems.size(), d_sys_,
                               d_traj_ + s * n, C * n, 3, stream_),
                 "gather launch");
@@ -883,11 +875,12 @@ void Engine::record(int id, int64_t nsteps, double* x, double* y, double* z,
   finish_steps({id}, nsteps);
 }
 
-// The same loop as step(), with the launches of two consecutive steps
-// (lists A then B) captured once into a HIP graph and replayed.  The graph
-// only references the device work-list buffer and counts, so it stays valid
-// while the composition (and therefore every count) is unchanged; the list
-// contents are re-uploaded on every call.
+// The same loop as step(), with the launches of one step of each snapshot
+// parity (lists A and B) captured once into two HIP graphs and replayed
+// alternately; target orderings due between steps run between replays.
+// The graphs only reference the device work-list buffer and counts, so they
+// stay valid while the composition (and therefore every count) is unchanged;
+// the list contents are re-uploaded on every call.
 void Engine::step_graph(const std::vector<int>& ids, int64_t nsteps) {
   if (nsteps < 0) throw Error(NBODY_E_INVALID, "nsteps < 0");
   if (gravity_ != GRAV_DIRECT)
@@ -895,35 +888,123 @@ void Engine::step_graph(const std::vector<int>& ids, int64_t nsteps) {
   for (int id : ids) check_id(id);
   if (nsteps == 0 || ids.empty()) return;
   prepare(ids);
+  const std::vector<SortPass> plan = plan_sorts(ids, nsteps);
+  size_t next = 0;
+  run_sorts(plan, next, 0);
   prime(ids);
   StepLists L;
   upload_step_lists(ids, L);
   int64_t key = (int64_t)(uintptr_t)d_step_items_ * 31 + mode_;
   for (int k = 0; k < 16; ++k) key = key * 1000003 + L.cnt[k];
-  if (key != gkey_ || ids != gids_ || !gexec_[0]) {
+  if (key != gkey_ || ids != gids_ || !gexec_[0] || !gexec_[1]) {
     drop_graphs();
-    hipGraph_t g = nullptr;
-    hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
-              "hipStreamBeginCapture");
-    try {
-      launch_step(L, 0);
-      launch_step(L, 1);
-    } catch (...) {
-      hipStreamEndCapture(stream_, &g);
-      if (g) hipGraphDestroy(g);
-      throw;
+    for (int which = 0; which < 2; ++which) {
+      hipGraph_t g = nullptr;
+      hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
+                "hipStreamBeginCapture");
+      try {
+        launch_step(L, which);
+      } catch (...) {
+        hipStreamEndCapture(stream_, &g);
+        if (g) hipGraphDestroy(g);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
+      hipError_t e = hipGraphInstantiate(&gexec_[which], g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      hip_check(e, "hipGraphInstantiate");
     }
-    hip_check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
-    hipError_t e = hipGraphInstantiate(&gexec_[0], g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    hip_check(e, "hipGraphInstantiate");
     gkey_ = key;
     gids_ = ids;
   }
-  for (int64_t t = 0; t + 1 < nsteps; t += 2)
-    hip_check(hipGraphLaunch(gexec_[0], stream_), "hipGraphLaunch");
-  if (nsteps & 1) launch_step(L, 0);
+  for (int64_t t = 0; t < nsteps; ++t) {
+    run_sorts(plan, next, t);
+    hip_check(hipGraphLaunch(gexec_[t & 1], stream_), "hipGraphLaunch");
+  }
   finish_steps(ids, nsteps);
+}
+
+// Target-ordering passes of one call (see SortPass).  Runs only in
+// MODE_F32R with the direct sum; otherwise the plan is empty.
+std::vector<Engine::SortPass> Engine::plan_sorts(const std::vector<int>& ids,
+                                                 int64_t nsteps) {
+  std::vector<SortPass> plan;
+  if (mode_ != MODE_F32R || gravity_ != GRAV_DIRECT || ids.empty()) return plan;
+  std::vector<SortSys> tab;
+  std::vector<int64_t> last(ids.size(), -2);
+  for (int64_t t = 0; t < nsteps; ++t) {
+    SortPass p{t, 0, 0, 0, (int)tab.size()};
+    for (size_t q = 0; q < ids.size(); ++q) {
+      const Sys& s = sys_[ids[q]];
+      const int64_t T = s.steps + t;
+      const int64_t prev = last[q] >= -1 ? last[q] : s.sorted_at;
+      const bool due = (t == 0 && prev < 0) || (T % kResortEvery == 0 && prev != T);
+      if (!due) continue;
+      last[q] = T;
+      SortSys e;
+      e.off = s.off;
+      e.n = (int32_t)s.p.n;
+      e.base = p.total;
+      const double time = (double)T * s.p.dt;  // the centre moves with vcom
+      e.ox = s.com[0] + s.vcom[0] * time;
+      e.oy = s.com[1] + s.vcom[1] * time;
+      e.oz = s.com[2] + s.vcom[2] * time;
+      e.inv = s.ext > 0.0 ? 1.0 / (4.0 * s.ext) : 1.0;
+      tab.push_back(e);
+      ++p.nsys;
+      p.total += e.n;
+      p.max_tiles = std::max<int>(p.max_tiles, (int)((e.n + kWave - 1) / kWave));
+    }
+    if (p.nsys > 0) plan.push_back(p);
+  }
+  if (plan.empty()) return plan;
+  for (size_t q = 0; q < ids.size(); ++q)
+    if (last[q] >= 0) sys_[ids[q]].sorted_at = last[q];
+  if (tab.size() > sort_tab_cap_) {
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    hipFree(d_sort_tab_);
+    d_sort_tab_ = nullptr;
+    sort_tab_cap_ = std::max(tab.size(), 2 * sort_tab_cap_);
+    hip_check(hipMalloc((void**)&d_sort_tab_, sizeof(SortSys) * sort_tab_cap_),
+              "hipMalloc(sort table)");
+  }
+  if (!d_keys_) {
+    const size_t len = (size_t)cap_ + kPad;
+    hip_check(hipMalloc((void**)&d_keys_, 2 * sizeof(uint64_t) * len),
+              "hipMalloc(sort keys)");
+    hip_check(hipMalloc((void**)&d_vals_, 2 * sizeof(int32_t) * len),
+              "hipMalloc(sort values)");
+    hip_check(sort_pairs(nullptr, sort_tmp_bytes_, nullptr, nullptr, nullptr,
+                         nullptr, (int)len, 64, stream_),
+              "radix sort sizing");
+    hip_check(hipMalloc(&d_sort_tmp_, sort_tmp_bytes_), "hipMalloc(sort scratch)");
+  }
+  // the table is host memory of this call: copy, then wait for the copy
+  hip_check(hipMemcpyAsync(d_sort_tab_, tab.data(), sizeof(SortSys) * tab.size(),
+                           hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(sort table)");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  return plan;
+}
+
+void Engine::run_sorts(const std::vector<SortPass>& plan, size_t& next,
+                       int64_t t) {
+  const size_t len = (size_t)cap_ + kPad;
+  for (; next < plan.size() && plan[next].t == t; ++next) {
+    const SortPass& p = plan[next];
+    int bits = 48;
+    while ((1 << (bits - 48)) < p.nsys) ++bits;  // rank bits above the code
+    hip_check(launch_sort_keys(d_arena_, d_sort_tab_ + p.tab, p.nsys,
+                               p.max_tiles, d_keys_, d_vals_, stream_),
+              "sort keys launch");
+    size_t bytes = sort_tmp_bytes_;
+    hip_check(sort_pairs(d_sort_tmp_, bytes, d_keys_, d_keys_ + len, d_vals_,
+                         d_vals_ + len, p.total, bits, stream_),
+              "radix sort");
+    hip_check(launch_sort_scatter(d_arena_, d_sort_tab_ + p.tab, p.nsys,
+                                  p.max_tiles, d_vals_ + len, stream_),
+              "sort scatter launch");
+  }
 }
 
 }  // namespace nbody

@@ -71,9 +71,6 @@ class Engine {
   const SystemParams& params(int id) const;
   int64_t steps_done(int id) const;
   void set_params(int id, const SystemParams& p);
-  // Re-centre the f32 origin of a system whenever its step count is a
-  // multiple of `every` (default 1: at every step call).
-  void set_rebase_interval(int id, int64_t every);
 
   // Advances every listed system by nsteps of its own integrator.
   void step(const std::vector<int>& ids, int64_t nsteps);
@@ -117,11 +114,25 @@ class Engine {
     bool vv_primed = false;
     bool prepared = false;
     int64_t steps = 0;
-    int64_t rebase_every = 1;
     int split = 1;              // source-range parts (fixed at upload)
-    double com[3] = {0, 0, 0};  // centre of mass at upload
-    double vcom[3] = {0, 0, 0};  // centre-of-mass velocity
+    double com[3] = {0, 0, 0};  // centre of mass at upload (sort box centre,
+    double vcom[3] = {0, 0, 0};  // advanced with the centre-of-mass velocity)
+    double ext = 0.0;           // largest coordinate distance from com
+    int64_t sorted_at = -1;     // step count of the last target ordering
   };
+  // MODE_F32R target ordering.  A system's targets are put in Hilbert order
+  // before its first F32C evaluation and before every step whose count is a
+  // multiple of kResortEvery -- a schedule of the system's own step count,
+  // so results do not depend on how steps are split across calls or on the
+  // other systems stepped with it.  plan_sorts() uploads the tables of every
+  // pass one call needs (pass t runs before that call's step t).
+  struct SortPass {
+    int64_t t;
+    int nsys, max_tiles, total, tab;
+  };
+  static constexpr int64_t kResortEvery = 32;
+  std::vector<SortPass> plan_sorts(const std::vector<int>& ids, int64_t nsteps);
+  void run_sorts(const std::vector<SortPass>& plan, size_t& next, int64_t t);
 
   size_t carve(char* base);
   void alloc_arena();
@@ -129,7 +140,6 @@ class Engine {
   void upload_table();
   void ensure_items(size_t n);
   void prepare(const std::vector<int>& ids);
-  void set_origin(Sys& s, SysDesc& d) const;
   SysDesc desc_of(int id) const;
   void tiles_of(int id, int stage, int src, std::vector<WorkItem>& out,
                 std::vector<WorkItem>* combine = nullptr) const;
@@ -177,6 +187,13 @@ class Engine {
   size_t stage_cap_ = 0;        // doubles
   double* d_traj_ = nullptr;    // trajectory buffer of record()
   size_t traj_cap_ = 0;         // doubles
+  // target ordering (MODE_F32C): keys/values in and out, sort scratch
+  SortSys* d_sort_tab_ = nullptr;
+  size_t sort_tab_cap_ = 0;
+  uint64_t* d_keys_ = nullptr;  // 2 * cap: in, out
+  int32_t* d_vals_ = nullptr;   // 2 * cap: in, out
+  void* d_sort_tmp_ = nullptr;
+  size_t sort_tmp_bytes_ = 0;
   std::vector<Sys> sys_;
   std::map<int64_t, int64_t> free_;  // offset -> length
   bool table_dirty_ = true;
@@ -188,8 +205,8 @@ class Engine {
   std::vector<int64_t> step_sig_;
   StepLists step_lists_;
 
-  // graph cache for step_graph
-  hipGraphExec_t gexec_[1] = {nullptr};
+  // graph cache for step_graph: one step of each snapshot parity
+  hipGraphExec_t gexec_[2] = {nullptr, nullptr};
   std::vector<int> gids_;
   int64_t gkey_ = -1;
   void drop_graphs();

@@ -28,6 +28,18 @@ hipError_t launch_diag(const Arena* A, const SysDesc* sys, const int32_t* list,
 hipError_t launch_gather(const Arena* A, const WorkItem* items, int nitems,
                          const SysDesc* sys, double* stage, int64_t total,
                          int nfields, hipStream_t s);
+// MODE_F32C target order: keys (rank << 48 | Hilbert index) and local indices
+// of the listed systems' bodies, a radix sort (sort_pairs: hipCUB; query the
+// temporary size with tmp == nullptr), then perm from the sorted values.
+hipError_t launch_sort_keys(const Arena* A, const SortSys* tab, int nsys,
+                            int max_tiles, uint64_t* keys, int32_t* vals,
+                            hipStream_t s);
+hipError_t launch_sort_scatter(const Arena* A, const SortSys* tab, int nsys,
+                               int max_tiles, const int32_t* vals,
+                               hipStream_t s);
+hipError_t sort_pairs(void* tmp, size_t& bytes, const uint64_t* kin,
+                      uint64_t* kout, const int32_t* vin, int32_t* vout, int n,
+                      int end_bit, hipStream_t s);
 hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s);
 hipError_t launch_probe(int* out, hipStream_t s);
 

@@ -14,6 +14,7 @@
 // running launch ever reads a value this launch writes.
 #include <hip/hip_runtime.h>
 
+#include <hipcub/hipcub.hpp>
 #include <type_traits>
 
 #include "device.h"
@@ -142,24 +143,6 @@ struct SrcF64 {  // f64 positions and g*m
   }
 };
 
-struct SrcF32 {  // f32 positions re-centred on the system origin, f32 g*m
-  const cf32 *X, *Y, *Z, *G;
-  template <int U>
-  struct Buf {
-    float x[U], y[U], z[U], g[U];
-  };
-  template <int U>
-  __device__ __forceinline__ void load(int64_t j, Buf<U>& b) const {
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      b.x[u] = X[j + u];
-      b.y[u] = Y[j + u];
-      b.z[u] = Z[j + u];
-      b.g[u] = G[j + u];
-    }
-  }
-};
-
 struct SrcX64 {  // f64 positions, f32 g*m
   const cf64 *X, *Y, *Z;
   const cf32* G;
@@ -263,11 +246,9 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
   s = t;
 }
 
-// Src = SrcX64 (S = double): f64 positions, each difference rounded once
-// to f32 -- exact to f64 rounding at any distance from the origin, so close
-// pairs keep full f32 accuracy.  Src = SrcF32 (S = float): positions of a
-// re-centred f32 snapshot (faster; close pairs far from the origin lose
-// accuracy to the rounding of their absolute positions).
+// MODE_F32C: f64 positions, each difference rounded once to f32 -- exact
+// to f64 rounding at any distance from the origin, so close pairs keep full
+// f32 accuracy.
 template <typename Src, typename S, int U>
 struct ChunkF32 {
   int64_t i, end;
@@ -301,6 +282,77 @@ struct ChunkF32 {
   }
 };
 
+#ifndef NBODY_LDS_BATCH
+#define NBODY_LDS_BATCH 8
+#endif
+constexpr int kLdsBatch = NBODY_LDS_BATCH;
+
+// v of lane L (an inactive lane holds its clamped body: any body will do).
+__device__ __forceinline__ double lane_f64(double v, int L) {
+  const int64_t b = __builtin_bit_cast(int64_t, v);
+  const int32_t lo = __builtin_amdgcn_readlane((int32_t)b, L);
+  const int32_t hi = __builtin_amdgcn_readlane((int32_t)(b >> 32), L);
+  return __builtin_bit_cast(double, ((int64_t)hi << 32) | (uint32_t)lo);
+}
+
+// MODE_F32R sweep.  The wave's targets are 64 consecutive bodies along a
+// Hilbert curve (perm), so they sit close together; every coordinate is
+// taken relative to the wave's origin (its middle target, lane 32).  Each 64-source chunk is loaded once
+// per wave, one source per lane, differenced in f64 against the origin and
+// rounded once to f32, and staged in LDS; every lane then reads the sources
+// back as broadcasts, in batches, and the pair loop is pure f32.  The
+// rounding of a coordinate is relative to its distance from the origin:
+// small for the close pairs (near the wave's targets), relative to |dx| for
+// the far ones.  Group sums of 32 sources are added with Kahan compensation.
+// Sources past the system's end stage as zero mass.  NOMASK: eps2 > 0, so a
+// source at the target's own position (itself) contributes exactly zero;
+// otherwise pairs with r2 == 0 are masked (coincident bodies then add
+// nothing, where the reference's f64 path would produce NaN).  Faster than
+// MODE_F32C (no f64 work per pair), but a coordinate's rounding is relative
+// to its distance from the origin, so a close pair far from its wave's
+// origin loses accuracy (about 2^-24 * |x - origin| / |dx| of that term).
+template <bool NOMASK>
+__device__ __forceinline__ void sweep_f32_lds(
+    float4* stage, const double* X, const double* Y, const double* Z,
+    const float* G, int64_t jlo, int64_t jhi, int64_t end, double ox,
+    double oy, double oz, float xi, float yi, float zi, float eps2,
+    Kahan3& acc) {
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  for (int64_t j0 = jlo; j0 < jhi; j0 += kWave) {
+    const int64_t j = j0 + lane;
+    float4 v = {0.f, 0.f, 0.f, 0.f};
+    if (j < end)
+      v = {(float)(X[j] - ox), (float)(Y[j] - oy), (float)(Z[j] - oz), G[j]};
+    stage[lane] = v;
+    __syncthreads();
+#pragma unroll 1
+    for (int q = 0; q < kWave; q += 32) {
+      float px = 0.f, py = 0.f, pz = 0.f;
+#pragma unroll
+      for (int u0 = 0; u0 < 32; u0 += kLdsBatch) {
+        float4 s[kLdsBatch];  // issue the batch's reads before any use
+#pragma unroll
+        for (int u = 0; u < kLdsBatch; ++u) s[u] = stage[q + u0 + u];
+#pragma unroll
+        for (int u = 0; u < kLdsBatch; ++u) {
+          const float dx = s[u].x - xi, dy = s[u].y - yi, dz = s[u].z - zi;
+          const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+          const float inv = __builtin_amdgcn_rsqf(r2);
+          float k = s[u].w * ((inv * inv) * inv);
+          if (!NOMASK) k = r2 > 0.f ? k : 0.f;
+          px = fmaf(k, dx, px);
+          py = fmaf(k, dy, py);
+          pz = fmaf(k, dz, pz);
+        }
+      }
+      kahan_add(acc.sx, acc.cx, px);
+      kahan_add(acc.sy, acc.cy, py);
+      kahan_add(acc.sz, acc.cz, pz);
+    }
+    __syncthreads();
+  }
+}
+
 template <int MODE, bool POT>
 __global__ __launch_bounds__(kBlock) void k_force(const Arena* Ap,
                                                   const WorkItem* items,
@@ -314,39 +366,50 @@ __global__ __launch_bounds__(kBlock) void k_force(const Arena* Ap,
   const CSys* sd = (const CSys*)(sysd + it.sys);
   const int64_t off = sd->off, n = sd->n, end = off + n;
   const int64_t t0 = off + (int64_t)it.tile * kWave;
-  const int64_t i = t0 + (int64_t)(threadIdx.x & (kWave - 1));
-  const bool active = i < end;
-  const int64_t il = active ? i : end - 1;
+  const int64_t slot = t0 + (int64_t)(threadIdx.x & (kWave - 1));
+  const bool active = slot < end;
+  // target body: the tile slot itself, or in MODE_F32R the slot's body in
+  // Hilbert order
+  const int64_t il = MODE == MODE_F32R ? off + A->perm[active ? slot : end - 1]
+                                       : (active ? slot : end - 1);
+  const int64_t i = MODE == MODE_F32R ? il : slot;
   const int src = it.src;
   double ax = 0.0, ay = 0.0, az = 0.0;
 
   const int nsplit = sd->nsplit, part = it.part;
-  if (MODE == MODE_F32C || MODE == MODE_F32R) {
+  if (MODE == MODE_F32R) {
+    __shared__ float4 stage[kWavesPerBlock][kWave];
+    const double ox = lane_f64(A->sx[src][il], 32);  // the middle target
+    const double oy = lane_f64(A->sy[src][il], 32);
+    const double oz = lane_f64(A->sz[src][il], 32);
+    const float xi = (float)(A->sx[src][il] - ox);
+    const float yi = (float)(A->sy[src][il] - oy);
+    const float zi = (float)(A->sz[src][il] - oz);
+    Kahan3 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int64_t jlo, jhi;
+    part_range<kWave>(off, n, nsplit, part, jlo, jhi);
+    const float e2 = sd->eps2f;
+    if (e2 > 0.f)
+      sweep_f32_lds<true>(stage[wave], A->sx[src], A->sy[src], A->sz[src],
+                          A->gm32, jlo, jhi, end, ox, oy, oz, xi, yi, zi, e2,
+                          acc);
+    else
+      sweep_f32_lds<false>(stage[wave], A->sx[src], A->sy[src], A->sz[src],
+                           A->gm32, jlo, jhi, end, ox, oy, oz, xi, yi, zi, e2,
+                           acc);
+    ax = (double)acc.sx - (double)acc.cx;
+    ay = (double)acc.sy - (double)acc.cy;
+    az = (double)acc.sz - (double)acc.cz;
+  } else if (MODE == MODE_F32C) {
     constexpr int U = kU32, KC = kKC;
-    using Src = typename std::conditional<MODE == MODE_F32C, SrcX64,
-                                          SrcF32>::type;
-    using S = typename std::conditional<MODE == MODE_F32C, double,
-                                        float>::type;
-    Src src_bundle;
-    S xi, yi, zi;
-    if constexpr (MODE == MODE_F32C) {
-      src_bundle = SrcX64{(const cf64*)A->sx[src], (const cf64*)A->sy[src],
-                          (const cf64*)A->sz[src], (const cf32*)A->gm32};
-      xi = A->sx[src][il];
-      yi = A->sy[src][il];
-      zi = A->sz[src][il];
-    } else {
-      src_bundle = SrcF32{(const cf32*)A->fx[src], (const cf32*)A->fy[src],
-                          (const cf32*)A->fz[src], (const cf32*)A->gm32};
-      xi = A->fx[src][il];
-      yi = A->fy[src][il];
-      zi = A->fz[src][il];
-    }
+    const SrcX64 src_bundle{(const cf64*)A->sx[src], (const cf64*)A->sy[src],
+                            (const cf64*)A->sz[src], (const cf32*)A->gm32};
+    const double xi = A->sx[src][il], yi = A->sy[src][il], zi = A->sz[src][il];
     Kahan3 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float px = 0.f, py = 0.f, pz = 0.f;
     int64_t jlo, jhi;
     part_range<U>(off, n, nsplit, part, jlo, jhi);
-    ChunkF32<Src, S, U> ch{i, end, xi, yi, zi, sd->eps2f, px, py, pz};
+    ChunkF32<SrcX64, double, U> ch{i, end, xi, yi, zi, sd->eps2f, px, py, pz};
     drive<U, KC>(src_bundle, jlo, jhi, t0, end, ch, [&] {
       kahan_add(acc.sx, acc.cx, px);
       kahan_add(acc.sy, acc.cy, py);
@@ -464,9 +527,6 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const Arena* Ap,
   A->sx[d][i] = x;
   A->sy[d][i] = y;
   A->sz[d][i] = z;
-  A->fx[d][i] = (float)(x - sd->ox);
-  A->fy[d][i] = (float)(y - sd->oy);
-  A->fz[d][i] = (float)(z - sd->oz);
 }
 
 // Per-system f64 diagnostics, one workgroup per listed system, fixed
@@ -560,6 +620,85 @@ inline unsigned blocks_for(int nitems) {
   return (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
 }
 
+// ---- MODE_F32C target order -------------------------------------------------
+__device__ __forceinline__ uint64_t spread16(uint32_t v) {  // 16 bits -> 48
+  uint64_t x = v & 0xffffu;
+  x = (x | (x << 16)) & 0x0000ff0000ffull;
+  x = (x | (x << 8)) & 0x00f00f00f00full;
+  x = (x | (x << 4)) & 0x0c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x249249249249ull;
+  return x;
+}
+
+__device__ __forceinline__ uint32_t quant16(double v, double o, double inv) {
+  const double q = fmin(fmax((v - o) * inv + 0.5, 0.0), 0.99999);  // NaN -> 0
+  return (uint32_t)(q * 65536.0);
+}
+
+// 3-D Hilbert index of 16-bit coordinates: Skilling's transform from axes
+// to the transposed index (J. Skilling, "Programming the Hilbert curve",
+// AIP Conf. Proc. 707, 381, 2004), then the bits interleaved with axis 0
+// most significant.  Consecutive indices are adjacent cells, so runs of 64
+// bodies are compact (a Z-order curve jumps across octant boundaries, which
+// cut through the dense centre of a system).
+__device__ __forceinline__ uint64_t hilbert48(uint32_t a, uint32_t b,
+                                              uint32_t c) {
+  uint32_t X[3] = {a, b, c};
+  for (uint32_t Q = 1u << 15; Q > 1; Q >>= 1) {
+    const uint32_t P = Q - 1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (X[i] & Q) {
+        X[0] ^= P;  // invert
+      } else {      // exchange
+        const uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  X[1] ^= X[0];  // Gray encode
+  X[2] ^= X[1];
+  uint32_t t = 0;
+  for (uint32_t Q = 1u << 15; Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  X[0] ^= t;
+  X[1] ^= t;
+  X[2] ^= t;
+  return (spread16(X[0]) << 2) | (spread16(X[1]) << 1) | spread16(X[2]);
+}
+
+// One block of 64 per (tile, listed system): key = rank << 48 | Hilbert
+// index of the body's position in the system's box; value = local index.
+__global__ __launch_bounds__(kWave) void k_sort_keys(const Arena* Ap,
+                                                     const SortSys* tab,
+                                                     uint64_t* keys,
+                                                     int32_t* vals) {
+  const int r = blockIdx.y;
+  const SortSys t = tab[r];
+  const int l = (int)blockIdx.x * kWave + (int)threadIdx.x;
+  if (l >= t.n) return;
+  const CArena* A = (const CArena*)Ap;
+  const int64_t i = t.off + l;
+  const uint64_t code = hilbert48(quant16(A->x[i], t.ox, t.inv),
+                                  quant16(A->y[i], t.oy, t.inv),
+                                  quant16(A->z[i], t.oz, t.inv));
+  keys[t.base + l] = ((uint64_t)r << 48) | code;
+  vals[t.base + l] = l;
+}
+
+// Sorted keys are grouped by rank, so system r's order is slots
+// [base, base + n) of the sorted values.
+__global__ __launch_bounds__(kWave) void k_sort_scatter(const Arena* Ap,
+                                                        const SortSys* tab,
+                                                        const int32_t* vals) {
+  const SortSys t = tab[blockIdx.y];
+  const int k = (int)blockIdx.x * kWave + (int)threadIdx.x;
+  if (k >= t.n) return;
+  const CArena* A = (const CArena*)Ap;
+  A->perm[t.off + k] = vals[t.base + k];
+}
+
 }  // namespace
 
 hipError_t launch_force(int mode, const Arena* A, const WorkItem* items,
@@ -647,6 +786,31 @@ hipError_t launch_gather(const Arena* A, const WorkItem* items, int nitems,
   hipLaunchKernelGGL(k_gather, dim3(blocks_for(nitems)), dim3(kBlock), 0, s, A,
                      items, nitems, sys, stage, total, nfields);
   return hipGetLastError();
+}
+
+hipError_t launch_sort_keys(const Arena* A, const SortSys* tab, int nsys,
+                            int max_tiles, uint64_t* keys, int32_t* vals,
+                            hipStream_t s) {
+  if (nsys <= 0 || max_tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_keys, dim3(max_tiles, nsys), dim3(kWave), 0, s, A,
+                     tab, keys, vals);
+  return hipGetLastError();
+}
+
+hipError_t launch_sort_scatter(const Arena* A, const SortSys* tab, int nsys,
+                               int max_tiles, const int32_t* vals,
+                               hipStream_t s) {
+  if (nsys <= 0 || max_tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_scatter, dim3(max_tiles, nsys), dim3(kWave), 0, s,
+                     A, tab, vals);
+  return hipGetLastError();
+}
+
+hipError_t sort_pairs(void* tmp, size_t& bytes, const uint64_t* kin,
+                      uint64_t* kout, const int32_t* vin, int32_t* vout, int n,
+                      int end_bit, hipStream_t s) {
+  return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, kin, kout, vin, vout,
+                                            n, 0, end_bit, s);
 }
 
 hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s) {

@@ -4,8 +4,7 @@
 //   m, gm64 (= G*m), gm32         bodies' masses, G folded in once per system
 //   x, y, z, vx, vy, vz           master state, binary64
 //   sx/sy/sz[2]                   f64 source snapshot, double-buffered
-//   fx/fy/fz[2]                   f32 source snapshot re-centred on the centre of
-//                                 mass (MODE_F32R), double-buffered
+//   perm                          MODE_F32R target order (Hilbert curve)
 //   ax, ay, az                    carried acceleration (velocity Verlet) or the
 //                                 output of a plain force evaluation
 //   rk[9]                         RK4 running sums and next-stage velocity
@@ -36,8 +35,10 @@ enum Mode : int32_t {            // force arithmetic (public NBODY_PRECISION_*)
   MODE_F32C = 2,                 // f64 positions, each difference rounded
                                  // once to f32, f32 pair terms, compensated
                                  // accumulation
-  MODE_F32R = 3,                 // f32 positions re-centred on the centre of
-                                 // mass, f32 pair terms, compensated
+  MODE_F32R = 3,                 // targets in Hilbert order, coordinates
+                                 // relative to each wave's middle target
+                                 // rounded to f32, sources staged in LDS,
+                                 // f32 pair terms, compensated accumulation
 };
 
 enum Integ : int32_t { INTEG_EULER = 0, INTEG_VV = 1, INTEG_RK4 = 2 };
@@ -59,19 +60,17 @@ enum Stage : int32_t {
   STAGE_POT = 8,       // store potential sum_j G m_j / sqrt(r2 + eps2) in phi
 };
 
-struct SysDesc {       // one per resident system (80 B)
+struct SysDesc {       // one per resident system (48 B)
   int64_t off;         // first body in the arena
   int32_t n;           // body count
   int32_t integ;       // Integ
   double dt;
   double eps2;         // softening squared, f64
   double g;            // gravitational constant
-  double ox, oy, oz;   // origin of the f32 snapshot (f64)
   float eps2f;         // eps2 rounded to f32
   int32_t nsplit;      // source-range parts per tile (1: no split)
-  int32_t pad_[2];
 };
-static_assert(sizeof(SysDesc) == 80, "SysDesc layout");
+static_assert(sizeof(SysDesc) == 48, "SysDesc layout");
 
 struct WorkItem {      // one wave: 64 consecutive targets of one system
   int32_t sys;         // index into the SysDesc table
@@ -88,13 +87,23 @@ struct Arena {         // device pointers, passed by value to every kernel
   float *gm32;
   double *x, *y, *z, *vx, *vy, *vz;
   double *sx[2], *sy[2], *sz[2];
-  float *fx[2], *fy[2], *fz[2];     // f32 snapshot: x - origin (MODE_F32R)
   double *ax, *ay, *az;
   double *rk[9];
   double *phi;
   double *px, *py, *pz; // per-part partial sums: [part * pstride + i]
+  int32_t* perm;        // MODE_F32R target order: tile slot off + k holds
+                        // body off + perm[off + k] (Hilbert order)
   int64_t pstride;     // cap + kPad
   int64_t cap;         // bodies (excluding the kPad slack)
+};
+
+// One system of a target-ordering pass (MODE_F32R): its bodies' Hilbert
+// codes are keyed (rank << 48 | code) so one sort orders every system.
+struct SortSys {
+  int64_t off;         // arena offset
+  int32_t n, base;     // bodies; first slot in the pass's key arrays
+  double ox, oy, oz;   // quantisation origin (the system's moving centre)
+  double inv;          // 1 / box width
 };
 
 // Per-system diagnostics written by the reduction kernel (9 doubles).

@@ -7,8 +7,9 @@
 // stops behind it (no starvation).  All resident systems advance together in
 // epochs; an epoch ends when some system reaches a snapshot boundary (a
 // multiple of snapshot_every of its own steps) or its last step, so every
-// system is snapshotted, re-centred and retired at the same own-step counts
-// whether it runs alone or in a mix.  New requests join at the next epoch.
+// system is snapshotted and retired at the same own-step counts whether it
+// runs alone or in a mix (its target reordering in MODE_F32R also follows
+// its own step count).  New requests join at the next epoch.
 // Snapshots are copied out after the epoch and queued as events; when the
 // queued snapshot bytes exceed the configured cap the worker pauses stepping
 // until the client drains events (backpressure).
@@ -295,7 +296,6 @@ class Server {
         try {
           a.sys = eng_->add(q.p, q.m.data(), q.x.data(), q.y.data(),
                             q.z.data(), q.vx.data(), q.vy.data(), q.vz.data());
-          eng_->set_rebase_interval(a.sys, cfg_.snapshot_every);
         } catch (const std::exception& ex) {
           Event e;
           fill(e.ev, NBODY_EVENT_FAILED, q, 0, a.t_admit, ex.what());

@@ -56,17 +56,16 @@ def test_record_equals_single_steps(gpu, prec, integ):
     im = nb.IntegratorMethod.from_str(integ)
     print(f"record vs steps: {prec.name} {integ} n=300", flush=True)
     rec = nb.GpuSimulation(b, params, integrator=im, precision=prec)
-    times, xs, ys, zs = rec.record(13, max_stage_bytes=4 * 3 * 300 * 8)  # fills of 4
-    assert np.array_equal(np.concatenate([[0.0], times]), times_of(1e-3, 13))
-    # one call of 13 steps: the same launches without the snapshots
+    nrec = 37  # crosses step 32, where F32R reorders its targets
+    times, xs, ys, zs = rec.record(nrec, max_stage_bytes=4 * 3 * 300 * 8)  # fills of 4
+    assert np.array_equal(np.concatenate([[0.0], times]), times_of(1e-3, nrec))
+    # one call of all steps: the same launches without the snapshots
     whole = nb.GpuSimulation(b, params, integrator=im, precision=prec)
-    whole.step(13)
+    whole.step(nrec)
     st = whole.snapshot()
     assert np.array_equal(xs[-1], st.pos_x) and np.array_equal(zs[-1], st.pos_z)
-    if prec == nb.Precision.F32R:
-        return  # F32R re-centres its f32 origin at every step call
     one = nb.GpuSimulation(b, params, integrator=im, precision=prec)
-    for s in range(13):
+    for s in range(nrec):
         one.step(1)
         st = one.snapshot()
         assert np.array_equal(xs[s], st.pos_x) and np.array_equal(ys[s], st.pos_y)

@@ -126,7 +126,7 @@ def test_config2_size_f64_and_f32c(gpu):
     assert e32.max() <= 1e-5
     # the re-centred f32 snapshot form: rounding of absolute positions limits
     # close pairs away from the centre (measured 5.4e-5 max here)
-    assert e32r.max() <= 1e-4 and np.median(e32r) <= 1e-7
+    assert e32r.max() <= 5e-5 and np.median(e32r) <= 1e-7
 
 
 @pytest.mark.parametrize("integ", INTEG)
@@ -263,19 +263,21 @@ def _mixed_systems(count, seed):
     return systems, integ
 
 
-@pytest.mark.parametrize("prec", [nb.Precision.F64, nb.Precision.F32C])
+@pytest.mark.parametrize("prec", [nb.Precision.F64, nb.Precision.F32C,
+                                  nb.Precision.F32R])
 def test_batch_alone_vs_mixed_bitwise(gpu, prec):
     systems, integ = _mixed_systems(24, 3)
     dts = [1e-3 if s % 2 else 5e-4 for s in range(len(systems))]
     mixed = nb.Batch(gpu, systems, integ, dts, 1.0, 1e-4, prec)
     mixed.step(5)
-    mixed.step(6)
+    mixed.step(30)     # crosses step 32: F32R reorders its targets there
     mx = mixed.download()
     for s in (0, 5, 13, 23):
-        # same call pattern: the f32 origin is re-centred at each step call
+        # another call pattern: every schedule follows the system's own step
+        # count, not the calls
         alone = nb.Batch(gpu, [systems[s]], [integ[s]], [dts[s]], 1.0, 1e-4, prec)
-        alone.step(5)
-        alone.step(6)
+        alone.step(20)
+        alone.step(15)
         ax = alone.download()
         o, n = mixed.offsets[s], mixed.n[s]
         for q in range(6):

@@ -101,3 +101,61 @@ def test_parts_cover_sources_once_in_order(u):
                             assert j < off + span + K_PAD
                         seen.extend(range(j0, j1))
                 assert seen == list(range(off, off + (n + u - 1) // u * u))
+
+
+# ---- MODE_F32R target order (kernels.hip hilbert48 / k_sort_keys) -----------
+def spread16(v):
+    v &= 0xffff
+    v = (v | (v << 16)) & 0x0000ff0000ff
+    v = (v | (v << 8)) & 0x00f00f00f00f
+    v = (v | (v << 4)) & 0x0c30c30c30c3
+    v = (v | (v << 2)) & 0x249249249249
+    return v
+
+
+def hilbert48(a, b, c):
+    """Line-for-line model of the device function (Skilling 2004)."""
+    X = [a, b, c]
+    Q = 1 << 15
+    while Q > 1:
+        P = Q - 1
+        for i in range(3):
+            if X[i] & Q:
+                X[0] ^= P
+            else:
+                t = (X[0] ^ X[i]) & P
+                X[0] ^= t
+                X[i] ^= t
+        Q >>= 1
+    X[1] ^= X[0]
+    X[2] ^= X[1]
+    t, Q = 0, 1 << 15
+    while Q > 1:
+        if X[2] & Q:
+            t ^= Q - 1
+        Q >>= 1
+    X = [v ^ t for v in X]
+    return (spread16(X[0]) << 2) | (spread16(X[1]) << 1) | spread16(X[2])
+
+
+def test_hilbert_order_is_a_curve_of_adjacent_cells():
+    """On a 16^3 grid of 16-bit coordinates the index is a bijection onto
+    0..4095 (in its top 12 bits) and consecutive indices are face-adjacent
+    cells: runs of bodies along it stay spatially compact."""
+    cells = {}
+    step = 1 << 12
+    for a in range(0, 1 << 16, step):
+        for b in range(0, 1 << 16, step):
+            for c in range(0, 1 << 16, step):
+                cells[hilbert48(a, b, c) >> 36] = (a // step, b // step, c // step)
+    keys = sorted(cells)
+    assert keys == list(range(4096))
+    for k0, k1 in zip(keys, keys[1:]):
+        p, q = cells[k0], cells[k1]
+        assert sum(abs(u - v) for u, v in zip(p, q)) == 1, (k0, p, q)
+
+
+def test_sort_keys_leave_room_for_the_rank():
+    top = (1 << 16) - 1
+    assert hilbert48(top, top, top) < (1 << 48)
+    assert hilbert48(0, 0, 0) == 0
