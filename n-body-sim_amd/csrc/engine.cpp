@@ -58,6 +58,7 @@ Engine::~Engine() {
   hipFree(d_stage_);
   hipHostFree(h_stage_);
   hipFree(d_traj_);
+  hipFree(d_snap_items_);
   hipFree(d_sort_tab_);
   hipFree(d_keys_);
   hipFree(d_vals_);
@@ -638,13 +639,24 @@ void Engine::finish_steps(const std::vector<int>& ids, int64_t nsteps) {
 }
 
 void Engine::step(const std::vector<int>& ids, int64_t nsteps) {
+  run_steps(ids, nsteps, false, [](int64_t) {});
+}
+
+template <typename Hook>
+void Engine::run_steps(const std::vector<int>& ids, int64_t nsteps,
+                       bool graph, Hook&& after_step) {
   if (nsteps < 0) throw Error(NBODY_E_INVALID, "nsteps < 0");
+  if (graph && gravity_ != GRAV_DIRECT)
+    throw Error(NBODY_E_UNSUPPORTED, "graph stepping runs the direct sum only");
   for (int id : ids) check_id(id);
   if (nsteps == 0 || ids.empty()) return;
   prepare(ids);
   if (gravity_ == GRAV_BARNES_HUT) {
     prime(ids);
-    for (int64_t t = 0; t < nsteps; ++t) bh_step(ids, t);
+    for (int64_t t = 0; t < nsteps; ++t) {
+      bh_step(ids, t);
+      after_step(t);
+    }
     finish_steps(ids, nsteps);
     return;
   }
@@ -654,11 +666,140 @@ void Engine::step(const std::vector<int>& ids, int64_t nsteps) {
   prime(ids);
   StepLists L;
   upload_step_lists(ids, L);
+  if (graph) {
+    // one step of each snapshot parity (lists A and B) captured once into
+    // two HIP graphs and replayed alternately; they reference only the
+    // device work-list buffer and counts, so they stay valid while the
+    // composition (and therefore every count) is unchanged
+    int64_t key = (int64_t)(uintptr_t)d_step_items_ * 31 + mode_;
+    for (int k = 0; k < 16; ++k) key = key * 1000003 + L.cnt[k];
+    if (key != gkey_ || ids != gids_ || !gexec_[0] || !gexec_[1]) {
+      drop_graphs();
+      for (int which = 0; which < 2; ++which) {
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
+                  "hipStreamBeginCapture");
+        try {
+          launch_step(L, which);
+        } catch (...) {
+          hipStreamEndCapture(stream_, &g);
+          if (g) hipGraphDestroy(g);
+          throw;
+        }
+        hip_check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
+        hipError_t e = hipGraphInstantiate(&gexec_[which], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        hip_check(e, "hipGraphInstantiate");
+      }
+      gkey_ = key;
+      gids_ = ids;
+    }
+  }
   for (int64_t t = 0; t < nsteps; ++t) {
     run_sorts(plan, next, t);
-    launch_step(L, (int)(t & 1));
+    if (graph)
+      hip_check(hipGraphLaunch(gexec_[t & 1], stream_), "hipGraphLaunch");
+    else
+      launch_step(L, (int)(t & 1));
+    after_step(t);
   }
   finish_steps(ids, nsteps);
+}
+
+void Engine::ensure_stage(size_t need) {
+  if (need <= stage_cap_) return;
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  hipFree(d_stage_);
+  hipHostFree(h_stage_);
+  d_stage_ = nullptr;
+  h_stage_ = nullptr;
+  stage_cap_ = std::max(need, 2 * stage_cap_);
+  hip_check(hipMalloc((void**)&d_stage_, sizeof(double) * stage_cap_),
+            "hipMalloc(gather staging)");
+  hip_check(hipHostMalloc((void**)&h_stage_, sizeof(double) * stage_cap_,
+                          hipHostMallocDefault),
+            "hipHostMalloc(gather staging)");
+}
+
+const double* Engine::step_snapshots(const std::vector<int>& ids,
+                                     int64_t nsteps, bool graph,
+                                     const std::vector<Snap>& snaps,
+                                     std::vector<int64_t>& rows,
+                                     int64_t& total) {
+  rows.clear();
+  total = 0;
+  // gather lists, grouped by step: [first[t], first[t+1]) are step t's items
+  std::vector<Snap> order(snaps);
+  std::stable_sort(order.begin(), order.end(),
+                   [](const Snap& a, const Snap& b) { return a.t < b.t; });
+  std::vector<WorkItem> items;
+  std::vector<std::pair<int64_t, std::pair<int, int>>> at;  // t -> range
+  std::vector<int64_t> row_of(snaps.size());
+  for (const Snap& sn : order) {
+    check_id(sn.id);
+    if (sn.t < 0 || sn.t >= nsteps)
+      throw Error(NBODY_E_INVALID, "snapshot step outside the call");
+  }
+  for (size_t k = 0; k < order.size(); ++k) {
+    const Snap& sn = order[k];
+    if (at.empty() || at.back().first != sn.t)
+      at.push_back({sn.t, {(int)items.size(), (int)items.size()}});
+    const int64_t n = sys_[sn.id].p.n;
+    const int64_t tiles = (n + kWave - 1) / kWave;
+    for (int64_t t = 0; t < tiles; ++t) {
+      WorkItem w{sn.id, (int32_t)t, (int16_t)STAGE_ACC, (int8_t)0, (int8_t)0};
+      w.pad_ = (int32_t)total;
+      items.push_back(w);
+    }
+    at.back().second.second = (int)items.size();
+    rows.push_back(total);
+    total += n;
+  }
+  // rows in the caller's order
+  {
+    std::vector<int64_t> r(snaps.size());
+    std::vector<size_t> idx(snaps.size());
+    for (size_t k = 0; k < idx.size(); ++k) idx[k] = k;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+      return snaps[a].t < snaps[b].t;
+    });
+    for (size_t k = 0; k < idx.size(); ++k) r[idx[k]] = rows[k];
+    rows.swap(r);
+  }
+  if (total > INT32_MAX) throw Error(NBODY_E_INVALID, "snapshots too large");
+  if (!items.empty()) {
+    validate_items(items);
+    ensure_stage((size_t)total * 3);
+    if (items.size() > snap_items_cap_) {
+      hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+      hipFree(d_snap_items_);
+      d_snap_items_ = nullptr;
+      snap_items_cap_ = std::max(items.size(), 2 * snap_items_cap_);
+      hip_check(hipMalloc((void**)&d_snap_items_,
+                          sizeof(WorkItem) * snap_items_cap_),
+                "hipMalloc(snapshot lists)");
+    }
+    hip_check(hipMemcpyAsync(d_snap_items_, items.data(),
+                             sizeof(WorkItem) * items.size(),
+                             hipMemcpyHostToDevice, stream_),
+              "hipMemcpy(snapshot lists)");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  }
+  size_t next = 0;
+  run_steps(ids, nsteps, graph, [&](int64_t t) {
+    for (; next < at.size() && at[next].first == t; ++next) {
+      const int lo = at[next].second.first, hi = at[next].second.second;
+      hip_check(launch_gather(d_arena_, d_snap_items_ + lo, hi - lo, d_sys_,
+                              d_stage_, total, 3, stream_),
+                "gather launch");
+    }
+  });
+  if (total == 0) return nullptr;
+  hip_check(hipMemcpyAsync(h_stage_, d_stage_, sizeof(double) * 3 * total,
+                           hipMemcpyDeviceToHost, stream_),
+            "hipMemcpy(snapshots)");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  return h_stage_;
 }
 
 void Engine::accelerations(int id, double* ax, double* ay, double* az) {
@@ -769,19 +910,7 @@ const double* Engine::gather(const std::vector<int>& ids, int nfields,
   if (total == 0) return nullptr;
   if (total > INT32_MAX) throw Error(NBODY_E_INVALID, "gather too large");
   const size_t need = (size_t)total * nfields;
-  if (need > stage_cap_) {
-    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    hipFree(d_stage_);
-    hipHostFree(h_stage_);
-    d_stage_ = nullptr;
-    h_stage_ = nullptr;
-    stage_cap_ = std::max(need, 2 * stage_cap_);
-    hip_check(hipMalloc((void**)&d_stage_, sizeof(double) * stage_cap_),
-              "hipMalloc(gather staging)");
-    hip_check(hipHostMalloc((void**)&h_stage_, sizeof(double) * stage_cap_,
-                            hipHostMallocDefault),
-              "hipHostMalloc(gather staging)");
-  }
+  ensure_stage(need);
   validate_items(items);
   ensure_items(items.size());
   hip_check(hipMemcpyAsync(d_items_, items.data(), sizeof(WorkItem) * items.size(),
@@ -875,53 +1004,10 @@ void Engine::record(int id, int64_t nsteps, double* x, double* y, double* z,
   finish_steps({id}, nsteps);
 }
 
-// The same loop as step(), with the launches of one step of each snapshot
-// parity (lists A and B) captured once into two HIP graphs and replayed
-// alternately; target orderings due between steps run between replays.
-// The graphs only reference the device work-list buffer and counts, so they
-// stay valid while the composition (and therefore every count) is unchanged;
-// the list contents are re-uploaded on every call.
+// step() with the launches of one step of each snapshot parity captured
+// into HIP graphs and replayed (see run_steps).
 void Engine::step_graph(const std::vector<int>& ids, int64_t nsteps) {
-  if (nsteps < 0) throw Error(NBODY_E_INVALID, "nsteps < 0");
-  if (gravity_ != GRAV_DIRECT)
-    throw Error(NBODY_E_UNSUPPORTED, "graph stepping runs the direct sum only");
-  for (int id : ids) check_id(id);
-  if (nsteps == 0 || ids.empty()) return;
-  prepare(ids);
-  const std::vector<SortPass> plan = plan_sorts(ids, nsteps);
-  size_t next = 0;
-  run_sorts(plan, next, 0);
-  prime(ids);
-  StepLists L;
-  upload_step_lists(ids, L);
-  int64_t key = (int64_t)(uintptr_t)d_step_items_ * 31 + mode_;
-  for (int k = 0; k < 16; ++k) key = key * 1000003 + L.cnt[k];
-  if (key != gkey_ || ids != gids_ || !gexec_[0] || !gexec_[1]) {
-    drop_graphs();
-    for (int which = 0; which < 2; ++which) {
-      hipGraph_t g = nullptr;
-      hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
-                "hipStreamBeginCapture");
-      try {
-        launch_step(L, which);
-      } catch (...) {
-        hipStreamEndCapture(stream_, &g);
-        if (g) hipGraphDestroy(g);
-        throw;
-      }
-      hip_check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
-      hipError_t e = hipGraphInstantiate(&gexec_[which], g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
-      hip_check(e, "hipGraphInstantiate");
-    }
-    gkey_ = key;
-    gids_ = ids;
-  }
-  for (int64_t t = 0; t < nsteps; ++t) {
-    run_sorts(plan, next, t);
-    hip_check(hipGraphLaunch(gexec_[t & 1], stream_), "hipGraphLaunch");
-  }
-  finish_steps(ids, nsteps);
+  run_steps(ids, nsteps, true, [](int64_t) {});
 }
 
 // Target-ordering passes of one call (see SortPass).  Runs only in

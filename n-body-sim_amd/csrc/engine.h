@@ -74,6 +74,21 @@ class Engine {
 
   // Advances every listed system by nsteps of its own integrator.
   void step(const std::vector<int>& ids, int64_t nsteps);
+  // One snapshot of a stepping call: positions of system `id` after step t
+  // (0-based) of the call.
+  struct Snap {
+    int64_t t;
+    int id;
+  };
+  // step() (graph = false) or step_graph() that also gathers the listed
+  // snapshots on the device as the steps run -- a gather launch after each
+  // step that has some -- and copies them to pinned host memory in one
+  // transfer at the end.  Returns a host view valid until the next gather:
+  // coordinate f of snapshot k starts at base[f * total + rows[k]].
+  // Synchronous.
+  const double* step_snapshots(const std::vector<int>& ids, int64_t nsteps,
+                               bool graph, const std::vector<Snap>& snaps,
+                               std::vector<int64_t>& rows, int64_t& total);
   // F(x) of one system into the device acceleration arrays (and optionally
   // to the host).  Drops a velocity-Verlet carry of that system.
   void accelerations(int id, double* ax, double* ay, double* az);
@@ -158,6 +173,12 @@ class Engine {
   void upload_step_lists(const std::vector<int>& ids, StepLists& L);
   void launch_step(const StepLists& L, int which);
   void finish_steps(const std::vector<int>& ids, int64_t nsteps);
+  // The stepping loop of step() / step_graph(); after_step(t) (may be empty)
+  // runs on the host after step t's launches are queued.
+  template <typename Hook>
+  void run_steps(const std::vector<int>& ids, int64_t nsteps, bool graph,
+                 Hook&& after_step);
+  void ensure_stage(size_t doubles);
   void check_id(int id) const;
   // One Barnes-Hut force evaluation of a system with its stage epilogue.
   void bh_round(int id, int stage, int src);
@@ -185,6 +206,8 @@ class Engine {
   double* d_stage_ = nullptr;   // gather staging (device)
   double* h_stage_ = nullptr;   // gather staging (pinned host)
   size_t stage_cap_ = 0;        // doubles
+  WorkItem* d_snap_items_ = nullptr;  // gather lists of step_snapshots()
+  size_t snap_items_cap_ = 0;
   double* d_traj_ = nullptr;    // trajectory buffer of record()
   size_t traj_cap_ = 0;         // doubles
   // target ordering (MODE_F32C): keys/values in and out, sort scratch

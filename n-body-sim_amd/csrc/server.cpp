@@ -5,14 +5,15 @@
 // admission is first-fit in arrival order, and a request that does not fit
 // may be bypassed by later ones at most kMaxBypass times before admission
 // stops behind it (no starvation).  All resident systems advance together in
-// epochs; an epoch ends when some system reaches a snapshot boundary (a
-// multiple of snapshot_every of its own steps) or its last step, so every
-// system is snapshotted and retired at the same own-step counts whether it
-// runs alone or in a mix (its target reordering in MODE_F32R also follows
-// its own step count).  New requests join at the next epoch.
-// Snapshots are copied out after the epoch and queued as events; when the
-// queued snapshot bytes exceed the configured cap the worker pauses stepping
-// until the client drains events (backpressure).
+// epochs of up to snapshot_every steps; an epoch ends early only at some
+// system's last step.  Snapshots are taken at multiples of snapshot_every
+// of each system's own steps, by gather launches between the epoch's steps,
+// so every system is snapshotted and retired at the same own-step counts
+// whether it runs alone or in a mix (its target reordering in MODE_F32R also
+// follows its own step count).  New requests join at the next epoch.  The
+// epoch's snapshots leave the device in one copy and are queued as events;
+// when the queued snapshot bytes exceed the configured cap the worker
+// pauses stepping until the client drains events (backpressure).
 #include "server.h"
 
 #include <hip/hip_runtime_api.h>
@@ -312,7 +313,8 @@ class Server {
         const uint64_t id = a.req.id;
         active_.emplace(id, std::move(a));
       }
-      // retire cancelled and finished requests, then choose the epoch length
+      // retire cancelled and finished requests, then choose the epoch length:
+      // up to snapshot_every steps, ending early only where a request ends
       retire_finished();
       std::vector<int> ids;
       int64_t k = cfg_.snapshot_every;
@@ -320,21 +322,35 @@ class Server {
         std::lock_guard<std::mutex> lk(mu_);
         for (auto& kv : active_) {
           const Active& a = kv.second;
-          const int64_t done = eng_->steps_done(a.sys);
           ids.push_back(a.sys);
-          k = std::min(k, cfg_.snapshot_every - done % cfg_.snapshot_every);
-          k = std::min(k, a.req.steps - done);
+          k = std::min(k, a.req.steps - eng_->steps_done(a.sys));
         }
       }
       if (ids.empty()) continue;
       std::sort(ids.begin(), ids.end());
+      // snapshots at own-step multiples of snapshot_every, gathered on the
+      // device as the epoch runs (the final state goes with DONE instead)
+      std::vector<Engine::Snap> snaps;
+      std::vector<const Active*> snap_req;
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (int64_t t = 0; t < k; ++t)
+          for (auto& kv : active_) {
+            const Active& a = kv.second;
+            const int64_t T = eng_->steps_done(a.sys) + t + 1;
+            if (a.req.want_snapshots && T < a.req.steps &&
+                T % cfg_.snapshot_every == 0) {
+              snaps.push_back({t, a.sys});
+              snap_req.push_back(&a);
+            }
+          }
+      }
+      std::vector<int64_t> rows;
+      int64_t total = 0;
+      const double* h = nullptr;
       try {
         // a graph pays off only while the resident set stays the same
-        if (ids == prev_ids)
-          eng_->step_graph(ids, k);
-        else
-          eng_->step(ids, k);
-        eng_->synchronize();
+        h = eng_->step_snapshots(ids, k, ids == prev_ids, snaps, rows, total);
       } catch (const std::exception& ex) {
         std::lock_guard<std::mutex> lk(mu_);
         for (auto it = active_.begin(); it != active_.end();) {
@@ -353,50 +369,27 @@ class Server {
         continue;
       }
       prev_ids = ids;
-      // snapshots at own-step multiples of snapshot_every (completions are
-      // retired at the top of the next iteration)
-      std::vector<int> snap_sys;
-      std::vector<const Active*> snap_req;
+      std::vector<Event> evs;
+      for (size_t q = 0; q < snaps.size(); ++q) {
+        const Active& a = *snap_req[q];
+        const int64_t n = a.req.p.n;
+        const int64_t T = eng_->steps_done(a.sys) - k + snaps[q].t + 1;
+        Event e;
+        fill(e.ev, NBODY_EVENT_SNAPSHOT, a.req, T, a.t_admit, "");
+        e.x.assign(h + rows[q], h + rows[q] + n);
+        e.y.assign(h + total + rows[q], h + total + rows[q] + n);
+        e.z.assign(h + 2 * total + rows[q], h + 2 * total + rows[q] + n);
+        evs.push_back(std::move(e));
+      }
       double inter = 0;
-      {
-        std::lock_guard<std::mutex> lk(mu_);
-        epoch_steps_ += k;
-        for (auto& kv : active_) {
-          const Active& a = kv.second;
-          const double n = (double)a.req.p.n;
-          inter += n * (n - 1.0) * (double)(k * evals_per_step(a.req.p.integ));
-          const int64_t done = eng_->steps_done(a.sys);
-          if (a.req.want_snapshots && done < a.req.steps &&
-              done % cfg_.snapshot_every == 0) {
-            snap_sys.push_back(a.sys);
-            snap_req.push_back(&a);
-          }
-        }
-        interactions_ += inter;
+      std::lock_guard<std::mutex> lk(mu_);
+      epoch_steps_ += k;
+      for (auto& kv : active_) {
+        const double n = (double)kv.second.req.p.n;
+        inter += n * (n - 1.0) * (double)(k * evals_per_step(kv.second.req.p.integ));
       }
-      if (!snap_sys.empty()) {
-        std::vector<int64_t> rows;
-        int64_t total = 0;
-        std::vector<Event> evs;
-        try {
-          const double* h = eng_->gather(snap_sys, 3, rows, total);
-          for (size_t q = 0; q < snap_sys.size(); ++q) {
-            const Active& a = *snap_req[q];
-            const int64_t n = a.req.p.n;
-            Event e;
-            fill(e.ev, NBODY_EVENT_SNAPSHOT, a.req, eng_->steps_done(a.sys),
-                 a.t_admit, "");
-            e.x.assign(h + rows[q], h + rows[q] + n);
-            e.y.assign(h + total + rows[q], h + total + rows[q] + n);
-            e.z.assign(h + 2 * total + rows[q], h + 2 * total + rows[q] + n);
-            evs.push_back(std::move(e));
-          }
-        } catch (const std::exception&) {
-          evs.clear();  // snapshots are best effort; the final state is not
-        }
-        std::lock_guard<std::mutex> lk(mu_);
-        for (auto& e : evs) push_event(std::move(e));
-      }
+      interactions_ += inter;
+      for (auto& e : evs) push_event(std::move(e));
     }
   }
 
