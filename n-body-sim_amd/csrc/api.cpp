@@ -465,6 +465,9 @@ int nbody_batch_create(nbody_ctx* ctx, int nsys, const int64_t* n,
     }
     bt->eng = std::make_unique<Engine>(ctx->device, ctx->stream,
                                        mode_of(precision), cap, nsys);
+    // hundreds of resident systems fill the GPU with waves already: the
+    // batched form keeps each system to at most 8 source parts
+    bt->eng->set_split_cap(8);
     for (int s = 0; s < nsys; ++s) {
       nbody::SystemParams p;
       p.n = n[s];

@@ -238,7 +238,7 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
   s.p = p;
   s.off = off;
   s.span = need;
-  s.split = split_for(p.n);
+  s.split = std::min(split_for(p.n), split_cap_);
   // centre of mass and its velocity, f64 in index order (the centre of the
   // MODE_F32R ordering box, advanced by vcom * elapsed time), and the
   // largest coordinate distance from it (the box size)

@@ -55,6 +55,12 @@ class Engine {
   // the fast f64 arithmetic in every other mode.
   void set_gravity(int gravity, double theta);
   int gravity() const { return gravity_; }
+  // Largest source-range split given to systems uploaded afterwards (the
+  // tuning table's per-size choice is clamped to it).  A system keeps its
+  // split for life, so its summation order never depends on what shares its
+  // launches; the cap lets an API form with many resident systems (already
+  // enough waves) use fewer, longer parts than one with a single system.
+  void set_split_cap(int cap) { split_cap_ = cap < 1 ? 1 : cap; }
   // Octree nodes of a system's last Barnes-Hut build (synchronises).
   int64_t tree_nodes(int id);
 
@@ -189,6 +195,7 @@ class Engine {
   hipStream_t stream_;
   int mode_;
   int gravity_ = GRAV_DIRECT;
+  int split_cap_ = kMaxSplit;
   double theta_ = 0.5;
   std::map<int, std::unique_ptr<BarnesHut>> trees_;
   int64_t cap_;

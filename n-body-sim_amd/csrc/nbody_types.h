@@ -26,7 +26,7 @@ constexpr int kWave = 64;        // gfx950 wavefront
 // dispatch of uneven work items); each wave runs one work item
 constexpr int kBlock = NBODY_BLOCK;
 constexpr int kPad = 128;        // slack elements after the arena
-constexpr int kMaxSplit = 8;     // source-range parts per target tile
+constexpr int kMaxSplit = 32;    // source-range parts per target tile
 
 enum Mode : int32_t {            // force arithmetic (public NBODY_PRECISION_*)
   MODE_F64_STRICT = 0,           // reference operation order, no contraction,

@@ -1,7 +1,7 @@
 """Autotuner for the split of source ranges across waves, per size bucket.
 
 For every bucket b (systems of n with floor(log2 n) == b, sampled at
-n = 2^b and n = 1.5 * 2^b), each candidate part count k in {1, 2, 4, 8} (with
+n = 2^b and n = 1.5 * 2^b), each candidate part count k in {1, 2, 4, 8, 16, 32} (with
 at least 64 sources per part) is timed on a Plummer sphere stepped with
 velocity Verlet in the f32-compensated and the f64 arithmetic.  The k with
 the smallest summed time wins.  The table is written as
@@ -23,7 +23,7 @@ if PKG not in sys.path:
 
 import nbody_amd as nb  # noqa: E402
 
-CANDIDATES = (1, 2, 4, 8)
+CANDIDATES = (1, 2, 4, 8, 16, 32)
 
 
 def time_case(gpu, bodies, prec, budget_ms=40.0):

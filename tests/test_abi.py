@@ -190,7 +190,7 @@ def test_bodies_length_check():
 
 def test_tuning_table_file_and_abi(tmp_path):
     """The split table loads from NBODY_AMD_TUNING (format `split b k`), is
-    clamped to 1..8 parts, and set/get round-trip through the C ABI."""
+    clamped to 1..32 parts, and set/get round-trip through the C ABI."""
     import subprocess
     import sys
     f = tmp_path / "t.txt"
@@ -203,7 +203,7 @@ def test_tuning_table_file_and_abi(tmp_path):
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
                          text=True, timeout=60)
     assert out.returncode == 0, out.stderr
-    assert out.stdout.split() == ["3", "8", "1", "2"]
+    assert out.stdout.split() == ["3", "32", "1", "2"]
 
 
 def test_committed_tuning_table_parses():
@@ -211,4 +211,4 @@ def test_committed_tuning_table_parses():
     path = os.path.join(nb._PKG_DIR, "tuning", "gfx950.txt")
     rows = [ln.split() for ln in open(path) if ln.startswith("split ")]
     assert len(rows) == 32
-    assert all(1 <= int(k) <= 8 for _, _, k in rows)
+    assert all(1 <= int(k) <= 32 for _, _, k in rows)
