@@ -300,8 +300,7 @@ __device__ __forceinline__ void add_pair(double gm, double dx, double dy,
     az = ok ? az + k * dz : az;
   } else {
     const double r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, eps2)));
-    const double inv = rsqrt_f64(r2);
-    const double k = ok ? gm * ((inv * inv) * inv) : 0.0;
+    const double k = ok ? gm * inv_cube_f64(r2) : 0.0;
     ax = fma(k, dx, ax);
     ay = fma(k, dy, ay);
     az = fma(k, dz, az);

@@ -38,20 +38,22 @@ __device__ __forceinline__ T load_uniform(const T* p) {
 // 1/sqrt(x) in binary64: v_rsq_f64 seed refined by one step of the
 // third-order series (1 - e)^(-1/2) = 1 + e/2 + 3e^2/8 + O(e^3) with
 // e = 1 - x*y^2 (Householder iteration of order 3).
-#ifndef NBODY_RSQ_SEED32
-#define NBODY_RSQ_SEED32 0
-#endif
 __device__ __forceinline__ double rsqrt_f64(double x) {
-#if NBODY_RSQ_SEED32
-  // seed from the f32 reciprocal square root (about 2^-23 relative); one
-  // third-order step takes it to below f64 rounding.  Requires x within the
-  // f32 normal range (about 1.2e-38 .. 3.4e38).
-  const double y = (double)__builtin_amdgcn_rsqf((float)x);
-#else
   const double y = __builtin_amdgcn_rsq(x);
-#endif
   const double e = fma(-x * y, y, 1.0);
   return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
+// x^(-3/2) in binary64 with the refinement folded into the cube: with the
+// v_rsq_f64 seed y and e = 1 - x*y^2 (|e| about 2^-22),
+//   (y (1 - e)^(-1/2))^3 = y^3 (1 - e)^(-3/2) = y^3 (1 + 1.5 e + O(e^2)),
+// so one fused multiply-add on t = y^3 replaces refining y before cubing
+// (the dropped e^2 term is below 2^-43 relative).
+__device__ __forceinline__ double inv_cube_f64(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  const double t = (y * y) * y;
+  return fma(t, 1.5 * e, t);
 }
 
 // Writes the next source snapshot of body i (f64 positions, every mode).

@@ -217,8 +217,7 @@ struct ChunkF64 {
         dz = ok ? dz : 0.0;
       }
       const double r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, eps2)));
-      const double inv = rsqrt_f64(r2);
-      double k = POT ? b.g[u] * inv : b.g[u] * ((inv * inv) * inv);
+      double k = b.g[u] * (POT ? rsqrt_f64(r2) : inv_cube_f64(r2));
       if (MASK) k = ok ? k : 0.0;
       if (POT) {
         ax += k;
