@@ -3,8 +3,11 @@
 Default workload (one GPU): a single N=65536 Plummer sphere (seed 2, scale
 radius 1, equal masses 1/N, G=1), softening 0.005, dt=5e-4, velocity Verlet
 with the acceleration carried between steps (one force evaluation per step).
-The headline `value` is the f32-compensated form; the plain f64 form is timed
-in the same run and reported beside it.  A "step" is one velocity-Verlet step
+The headline `value` is an f32-compensated form (--f32: f32r, per-wave
+relative coordinates, the default for one large system; or f32c, one f64
+difference per pair, the default for the batched and served mixes of small
+systems); the other f32 form and the plain f64 form are timed in the same run
+and reported beside it.  A "step" is one velocity-Verlet step
 of the whole system; interactions counted as N(N-1) per force evaluation and
 20 flops per interaction.  Inputs are resident in HBM before timing starts;
 the timed region is bracketed by HIP events on the library's stream.
@@ -116,61 +119,80 @@ def roofline(rate, dtype, kernel_ms, traffic=None):
             "kernel_avg_ms": kernel_ms}
 
 
+F32_FORMS = {"f32r": nb.Precision.F32R, "f32c": nb.Precision.F32C}
+DTYPE = {
+    "f32r": "f32 pair terms (per-wave relative coordinates; f64 differences "
+            "split in two f32 words for near chunks), f64 state",
+    "f32c": "f32 pair terms (f64 differences), f64 state",
+}
+
+
+def f32_forms(args):
+    """The headline f32 form first, then the other one."""
+    other = "f32c" if args.f32 == "f32r" else "f32r"
+    return [args.f32, other]
+
+
 def run_single(args, gpu):
     n, g, eps, dt = 65536, 1.0, 0.005, 5e-4
     b = nb.plummer(n, 2)
     inter = n * (n - 1) * args.steps
+    forms = f32_forms(args) + ["f64"]
+    precs = dict(F32_FORMS, f64=nb.Precision.F64)
     res = {}
-    for prec in (nb.Precision.F32C, nb.Precision.F64):
+    for name in forms:
+        prec = precs[name]
         log(f"timing {prec.name}: N={n} VV {args.steps} steps (+{args.warmup} warmup)")
-        ms, acc0, e0, e1, fin = time_single(gpu, b, prec, g, eps, dt, args.steps,
-                                            args.warmup)
-        res[prec] = (ms, acc0, e0, e1, fin)
-        log(f"  {ms:.1f} ms -> {inter / (ms / 1e3):.4e} int/s")
-    traffic = measured_traffic()
-    ms32, acc32, e0_32, e1_32, fin32 = res[nb.Precision.F32C]
-    ms64, acc64, e0_64, e1_64, fin64 = res[nb.Precision.F64]
-    dev = np.sqrt((fin32.pos_x - fin64.pos_x) ** 2 + (fin32.pos_y - fin64.pos_y) ** 2 +
-                  (fin32.pos_z - fin64.pos_z) ** 2)
-    rate32 = inter / (ms32 / 1e3)
-    rate64 = inter / (ms64 / 1e3)
+        res[name] = time_single(gpu, b, prec, g, eps, dt, args.steps, args.warmup)
+        log(f"  {res[name][0]:.1f} ms -> {inter / (res[name][0] / 1e3):.4e} int/s")
+    traffic = measured_traffic() or {}
+    head, alt = forms[0], forms[1]
+    rate = {k: inter / (res[k][0] / 1e3) for k in forms}
+    ms = {k: res[k][0] for k in forms}
     cpu = None
     acc_ref = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle newton-parallel at N={n} for ~{args.cpu_seconds} s")
         cpu, acc_ref = cpu_baseline(b, g, eps, args.cpu_seconds)
     acc_rep = {}
-    if acc_ref is not None:
-        acc_rep = {"f32c_max_rel": rel_err(acc32, acc_ref)[0],
-                   "f32c_median_rel": rel_err(acc32, acc_ref)[1],
-                   "f64_max_rel": rel_err(acc64, acc_ref)[0],
-                   "f64_median_rel": rel_err(acc64, acc_ref)[1]}
     drift = lambda e0, e1: abs(e1[2] - e0[2]) / abs(e0[2])  # noqa: E731
-    acc_rep["f32c_energy_drift"] = drift(e0_32, e1_32)
-    acc_rep["f64_energy_drift"] = drift(e0_64, e1_64)
-    # trajectories after warmup + steps: f32c against f64, in units of the
-    # Plummer scale radius (1)
-    acc_rep["trajectory_max_dev_f32c_vs_f64"] = float(dev.max())
-    acc_rep["trajectory_rms_dev_f32c_vs_f64"] = float(np.sqrt(np.mean(dev ** 2)))
+    fin64 = res["f64"][4]
+    for k in forms:
+        _, acc0, e0, e1, fin = res[k]
+        if acc_ref is not None:
+            acc_rep[f"{k}_max_rel"], acc_rep[f"{k}_median_rel"] = rel_err(acc0, acc_ref)
+        acc_rep[f"{k}_energy_drift"] = drift(e0, e1)
+        if k != "f64":
+            # trajectories after warmup + steps against f64, in units of the
+            # Plummer scale radius (1)
+            dev = np.sqrt((fin.pos_x - fin64.pos_x) ** 2 + (fin.pos_y - fin64.pos_y) ** 2 +
+                          (fin.pos_z - fin64.pos_z) ** 2)
+            acc_rep[f"trajectory_max_dev_{k}_vs_f64"] = float(dev.max())
+            acc_rep[f"trajectory_rms_dev_{k}_vs_f64"] = float(np.sqrt(np.mean(dev ** 2)))
     acc_rep["trajectory_steps"] = args.warmup + args.steps
     return {
         "metric": "pair-interactions/s (f32-compensated forces, f64 state)",
-        "value": rate32, "unit": "pair-interactions/s", "n_gpus": 1,
+        "value": rate[head], "unit": "pair-interactions/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms32 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair terms (f64 differences), f64 state",
+        "ms_per_step": ms[head] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[head],
         "data": "synthetic (seeded Plummer sphere, generated on the host)",
         "config": {"workload": "single Plummer sphere N=65536, softening 0.005, "
                                "dt 5e-4, velocity-verlet", "n": n, "g": g,
                    "softening": eps, "dt": dt, "integrator": "velocity-verlet",
-                   "global_batch": 1, "parallelism": "none (1 GPU)"},
-        "roofline": roofline(rate32, "f32", ms32 / args.steps,
-                             (traffic or {}).get("f32c")),
+                   "f32_form": head, "global_batch": 1,
+                   "parallelism": "none (1 GPU)"},
+        "roofline": roofline(rate[head], "f32", ms[head] / args.steps,
+                             traffic.get(head)),
         "cpu_baseline": cpu,
-        "fp64_plain": {"value": rate64, "ms_per_step": ms64 / args.steps,
-                       "roofline": roofline(rate64, "f64", ms64 / args.steps,
-                                            (traffic or {}).get("f64")),
-                       "f32c_speedup": rate32 / rate64},
+        "f32_alt": {"form": alt, "value": rate[alt], "dtype": DTYPE[alt],
+                    "ms_per_step": ms[alt] / args.steps,
+                    "roofline": roofline(rate[alt], "f32", ms[alt] / args.steps,
+                                         traffic.get(alt))},
+        "fp64_plain": {"value": rate["f64"], "ms_per_step": ms["f64"] / args.steps,
+                       "roofline": roofline(rate["f64"], "f64", ms["f64"] / args.steps,
+                                            traffic.get("f64")),
+                       "f32_speedup": rate[head] / rate["f64"]},
         "accuracy": acc_rep,
     }
 
@@ -183,8 +205,11 @@ def run_batched(args, gpu):
     integ = [s % 3 for s in range(nsys)]
     evals = np.array([1, 1, 4])[integ]
     inter = float(np.sum(ns * (ns - 1) * evals)) * args.steps
-    out = {}
-    for prec in (nb.Precision.F32C, nb.Precision.F64):
+    forms = f32_forms(args) + ["f64"]
+    precs = dict(F32_FORMS, f64=nb.Precision.F64)
+    rate = {}
+    for name in forms:
+        prec = precs[name]
         bt = nb.Batch(gpu, systems, integ, 1e-3, 1.0, 1e-6, prec)
         bt.step(args.warmup)
         gpu.synchronize()
@@ -192,26 +217,27 @@ def run_batched(args, gpu):
         bt.step(args.steps)
         gpu.record(1)
         ms = gpu.elapsed_ms(0, 1)
-        out[prec] = ms
-        log(f"batched {prec.name}: {ms:.1f} ms -> {inter / (ms / 1e3):.4e} int/s")
+        rate[name] = inter / (ms / 1e3)
+        log(f"batched {prec.name}: {ms:.1f} ms -> {rate[name]:.4e} int/s")
         bt.close()
-    ms32 = out[nb.Precision.F32C]
-    rate32 = inter / (ms32 / 1e3)
-    rate64 = inter / (out[nb.Precision.F64] / 1e3)
+    head, alt = forms[0], forms[1]
+    ms_head = inter / rate[head] * 1e3
     return {
         "metric": "pair-interactions/s (batched, f32-compensated forces)",
-        "value": rate32, "unit": "pair-interactions/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms32 / args.steps,
+        "value": rate[head], "unit": "pair-interactions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_head / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 pair terms (f64 differences), f64 state",
+        "dtype": DTYPE[head],
         "data": "synthetic (seeded star systems)",
         "config": {"workload": "512 star systems, N log-uniform [64, 8192], "
                                "euler/vv/rk4 1:1:1, dt 1e-3", "systems": nsys,
-                   "bodies": int(ns.sum()), "global_batch": nsys,
+                   "bodies": int(ns.sum()), "f32_form": head, "global_batch": nsys,
                    "parallelism": "none (1 GPU)"},
-        "roofline": roofline(rate32, "f32", ms32 / args.steps),
+        "roofline": roofline(rate[head], "f32", ms_head / args.steps),
         "cpu_baseline": None,
-        "fp64_plain": {"value": rate64, "f32c_speedup": rate32 / rate64},
+        "f32_alt": {"form": alt, "value": rate[alt]},
+        "fp64_plain": {"value": rate["f64"], "f32_speedup": rate[head] / rate["f64"]},
     }
 
 
@@ -235,7 +261,7 @@ def run_serve(args, gpu):
     results = []
     for rate in rates:
         srv = nb.Server(capacity_bodies=4 << 20, max_systems=4096, snapshot_every=10,
-                        precision=nb.Precision.F32C)
+                        precision=F32_FORMS[args.f32])
         gaps = rng.exponential(1.0 / rate, nreq)
         lat, done, interactions = [], [0], 0.0
         t_first = [None]
@@ -281,7 +307,7 @@ def run_serve(args, gpu):
         "value": best["solves_s"], "unit": "solves/s", "n_gpus": 1,
         "steps": None, "warmup": 0, "ms_per_step": None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 pair terms (f64 differences), f64 state",
+        "dtype": DTYPE[args.f32],
         "data": f"synthetic Poisson request stream ({args.mix.replace(',', ' / ')})",
         "config": {"workload": "served requests, N log-uniform [32, 16384], steps "
                                "uniform [10, 1000], random integrator, dt in "
@@ -296,7 +322,8 @@ def run_large(args, gpu):
     """Two colliding Plummer spheres, N=1,048,576, RK4 (4 evaluations/step)."""
     n, g, eps, dt = 1 << 20, 1.0, 0.002, 2.5e-4
     b = nb.colliding(n, 5, separation=10.0, rel_speed=0.5)
-    out = {}
+    forms = f32_forms(args) + ["f64"]
+    precs = dict(F32_FORMS, f64=nb.Precision.F64)
     # sampled-target oracle: 4096 random targets against all sources (f64 CPU)
     acc_rep = {}
     if not args.no_cpu_baseline:
@@ -306,15 +333,18 @@ def run_large(args, gpu):
         ref = oracle.newton_targets(b.masses, b.pos_x, b.pos_y, b.pos_z, g, eps, idx,
                                     min(16, oracle.default_threads()))
         log(f"sampled oracle: 4096 targets in {time.perf_counter() - t0:.1f} s")
-        for prec in (nb.Precision.F32C, nb.Precision.F64):
+        for name in forms:
             db = nb.DeviceBodies(gpu, b)
-            got = nb.GpuNewtonParallelGravity(g, eps * eps, prec).calculate_accelerations(gpu, db)
+            got = nb.GpuNewtonParallelGravity(g, eps * eps, precs[name]) \
+                .calculate_accelerations(gpu, db)
             db.close()
             mx, med = rel_err(tuple(a[idx] for a in got), ref)
-            acc_rep[f"{prec.name.lower()}_max_rel_sampled"] = mx
-            acc_rep[f"{prec.name.lower()}_median_rel_sampled"] = med
-    for prec in (nb.Precision.F32C, nb.Precision.F64):
-        grav = nb.GpuNewtonParallelGravity(g, eps * eps, prec)
+            acc_rep[f"{name}_max_rel_sampled"] = mx
+            acc_rep[f"{name}_median_rel_sampled"] = med
+    inter = n * (n - 1) * 4 * args.steps
+    rate, ms = {}, {}
+    for name in forms:
+        grav = nb.GpuNewtonParallelGravity(g, eps * eps, precs[name])
         integ = nb.GpuRungeKuttaIntegrator(dt)
         db = nb.DeviceBodies(gpu, b)
         e0 = energy(gpu, db, g, eps)
@@ -322,31 +352,31 @@ def run_large(args, gpu):
         gpu.record(0)
         integ.step(gpu, db, grav, args.steps)
         gpu.record(1)
-        ms = gpu.elapsed_ms(0, 1)
+        ms[name] = gpu.elapsed_ms(0, 1)
         e1 = energy(gpu, db, g, eps)
-        out[prec] = (ms, abs(e1[2] - e0[2]) / abs(e0[2]))
-        log(f"large {prec.name}: {ms:.0f} ms for {args.steps} RK4 steps, "
-            f"energy drift {out[prec][1]:.3e}")
+        rate[name] = inter / (ms[name] / 1e3)
+        acc_rep[f"{name}_energy_drift"] = abs(e1[2] - e0[2]) / abs(e0[2])
+        log(f"large {name}: {ms[name]:.0f} ms for {args.steps} RK4 steps, "
+            f"energy drift {acc_rep[f'{name}_energy_drift']:.3e}")
         db.close()
-    inter = n * (n - 1) * 4 * args.steps
-    ms32, drift32 = out[nb.Precision.F32C]
-    ms64, drift64 = out[nb.Precision.F64]
-    rate32, rate64 = inter / (ms32 / 1e3), inter / (ms64 / 1e3)
+    head, alt = forms[0], forms[1]
     return {
         "metric": "pair-interactions/s (f32-compensated forces, f64 state)",
-        "value": rate32, "unit": "pair-interactions/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": 0, "ms_per_step": ms32 / args.steps,
+        "value": rate[head], "unit": "pair-interactions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 0, "ms_per_step": ms[head] / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 pair terms (f64 differences), f64 state",
+        "dtype": DTYPE[head],
         "data": "synthetic (two seeded Plummer spheres)",
         "config": {"workload": "two colliding Plummer spheres N=1048576, softening "
                                "0.002, dt 2.5e-4, runge-kutta", "n": n,
-                   "parallelism": "none (1 GPU)"},
-        "roofline": roofline(rate32, "f32", ms32 / args.steps / 4),
+                   "f32_form": head, "parallelism": "none (1 GPU)"},
+        "roofline": roofline(rate[head], "f32", ms[head] / args.steps / 4),
         "cpu_baseline": None,
-        "fp64_plain": {"value": rate64, "f32c_speedup": rate32 / rate64,
-                       "roofline": roofline(rate64, "f64", ms64 / args.steps / 4)},
-        "accuracy": dict(acc_rep, f32c_energy_drift=drift32, f64_energy_drift=drift64),
+        "f32_alt": {"form": alt, "value": rate[alt],
+                    "roofline": roofline(rate[alt], "f32", ms[alt] / args.steps / 4)},
+        "fp64_plain": {"value": rate["f64"], "f32_speedup": rate[head] / rate["f64"],
+                       "roofline": roofline(rate["f64"], "f64", ms["f64"] / args.steps / 4)},
+        "accuracy": acc_rep,
     }
 
 
@@ -485,6 +515,11 @@ def main():
     ap.add_argument("--requests", type=int, default=150, help="serve: requests per rate")
     ap.add_argument("--mix", default="plummer,star",
                     help="serve: request scenarios, from plummer, star, disc")
+    ap.add_argument("--f32", choices=sorted(F32_FORMS), default=None,
+                    help="the f32 form of the headline value (the other one is "
+                         "timed beside it); default: the faster one for the "
+                         "workload (f32r for one large system, f32c for many "
+                         "small ones)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
@@ -494,6 +529,10 @@ def main():
               "system would shard by target range with one all-gather of source "
               "positions per force evaluation -- not built here)", file=sys.stderr)
         sys.exit(2)
+    if args.f32 is None:
+        # many small systems (a few chunks each, most of them near their
+        # targets) favour the per-pair f64 difference of f32c
+        args.f32 = "f32c" if args.workload in ("batched", "serve") else "f32r"
     if args.steps is None:
         args.steps = {"single": 200, "batched": 50, "serve": 0, "large": 2,
                       "bh": 5, "copy": 1}[args.workload]

@@ -395,7 +395,7 @@ __global__ __launch_bounds__(kWave) void k_bh_walk_strict(const Arena* Ap,
   }
   if (valid) {
     const int64_t i = sd->off + T.perm[k];
-    epilogue<MODE_F64_STRICT>(A, sd, stage, src ^ 1, i, ax, ay, az);
+    epilogue<MODE_F64_STRICT>(A, sd, stage, src ^ 1, i, i, ax, ay, az);
   }
 }
 
@@ -497,7 +497,7 @@ __global__ __launch_bounds__(kWave) void k_bh_walk_fast(const Arena* Ap,
   }
   if (valid) {
     const int64_t i = sd->off + T.perm[k];
-    epilogue<MODE_F64>(A, sd, stage, src ^ 1, i, ax, ay, az);
+    epilogue<MODE_F64>(A, sd, stage, src ^ 1, i, i, ax, ay, az);
   }
 }
 

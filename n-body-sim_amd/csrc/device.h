@@ -56,24 +56,31 @@ __device__ __forceinline__ double inv_cube_f64(double x) {
   return fma(t, 1.5 * e, t);
 }
 
-// Writes the next source snapshot of body i (f64 positions, every mode).
+// Writes the next source snapshot of body i (f64 positions, every mode);
+// MODE_F32R also writes it at the body's slot in Hilbert order.
 template <int MODE>
 __device__ __forceinline__ void put_snapshot(const CArena* A, int dst,
-                                             int64_t i, const CSys* sd,
+                                             int64_t i, int64_t slot,
                                              double x, double y, double z) {
-  (void)sd;
   A->sx[dst][i] = x;
   A->sy[dst][i] = y;
   A->sz[dst][i] = z;
+  if (MODE == MODE_F32R) {
+    A->hx[dst][slot] = x;
+    A->hy[dst][slot] = y;
+    A->hz[dst][slot] = z;
+  }
 }
 
-// Integrator stage for body i, given its fresh acceleration (ax, ay, az).
+// Integrator stage for body i (at `slot` in MODE_F32R's Hilbert order; any
+// value in the other modes), given its fresh acceleration (ax, ay, az).
 // Expression trees follow the reference exactly; contraction is off so the
 // stage arithmetic is bit-identical to the reference for identical inputs.
 template <int MODE>
 __device__ __forceinline__ void epilogue(const CArena* A, const CSys* sd,
                                          int stage, int dst, int64_t i,
-                                         double ax, double ay, double az) {
+                                         int64_t slot, double ax, double ay,
+                                         double az) {
 #pragma clang fp contract(off)
   const double dt = sd->dt;
   switch (stage) {
@@ -96,7 +103,7 @@ __device__ __forceinline__ void epilogue(const CArena* A, const CSys* sd,
       A->x[i] = x;
       A->y[i] = y;
       A->z[i] = z;
-      put_snapshot<MODE>(A, dst, i, sd, x, y, z);
+      put_snapshot<MODE>(A, dst, i, slot, x, y, z);
       break;
     }
     case STAGE_VV_PRIME: {  // velocity_verlet.rs:82-86, first half only
@@ -107,7 +114,7 @@ __device__ __forceinline__ void epilogue(const CArena* A, const CSys* sd,
       const double x = A->x[i] + ((A->vx[i] * dt) + ((0.5 * ax) * dt2));
       const double y = A->y[i] + ((A->vy[i] * dt) + ((0.5 * ay) * dt2));
       const double z = A->z[i] + ((A->vz[i] * dt) + ((0.5 * az) * dt2));
-      put_snapshot<MODE>(A, dst, i, sd, x, y, z);
+      put_snapshot<MODE>(A, dst, i, slot, x, y, z);
       break;
     }
     case STAGE_VV: {  // velocity_verlet.rs:82-86 then :97-101, a_n carried
@@ -129,7 +136,7 @@ __device__ __forceinline__ void epilogue(const CArena* A, const CSys* sd,
       A->ax[i] = ax;
       A->ay[i] = ay;
       A->az[i] = az;
-      put_snapshot<MODE>(A, dst, i, sd, x + ((vx * dt) + ((0.5 * ax) * dt2)),
+      put_snapshot<MODE>(A, dst, i, slot, x + ((vx * dt) + ((0.5 * ax) * dt2)),
                          y + ((vy * dt) + ((0.5 * ay) * dt2)),
                          z + ((vz * dt) + ((0.5 * az) * dt2)));
       break;
@@ -150,7 +157,7 @@ __device__ __forceinline__ void epilogue(const CArena* A, const CSys* sd,
       A->rk[6][i] = k2x;
       A->rk[7][i] = k2y;
       A->rk[8][i] = k2z;
-      put_snapshot<MODE>(A, dst, i, sd, A->x[i] + h * vx, A->y[i] + h * vy,
+      put_snapshot<MODE>(A, dst, i, slot, A->x[i] + h * vx, A->y[i] + h * vy,
                          A->z[i] + h * vz);
       break;
     }
@@ -168,7 +175,7 @@ __device__ __forceinline__ void epilogue(const CArena* A, const CSys* sd,
       A->rk[6][i] = k3x;
       A->rk[7][i] = k3y;
       A->rk[8][i] = k3z;
-      put_snapshot<MODE>(A, dst, i, sd, A->x[i] + h * k2x, A->y[i] + h * k2y,
+      put_snapshot<MODE>(A, dst, i, slot, A->x[i] + h * k2x, A->y[i] + h * k2y,
                          A->z[i] + h * k2z);
       break;
     }
@@ -181,7 +188,7 @@ __device__ __forceinline__ void epilogue(const CArena* A, const CSys* sd,
       A->rk[3][i] = A->rk[3][i] + (2.0 * ax);
       A->rk[4][i] = A->rk[4][i] + (2.0 * ay);
       A->rk[5][i] = A->rk[5][i] + (2.0 * az);
-      put_snapshot<MODE>(A, dst, i, sd, A->x[i] + dt * k3x, A->y[i] + dt * k3y,
+      put_snapshot<MODE>(A, dst, i, slot, A->x[i] + dt * k3x, A->y[i] + dt * k3y,
                          A->z[i] + dt * k3z);
       break;
     }
@@ -196,7 +203,7 @@ __device__ __forceinline__ void epilogue(const CArena* A, const CSys* sd,
       A->x[i] = x;
       A->y[i] = y;
       A->z[i] = z;
-      put_snapshot<MODE>(A, dst, i, sd, x, y, z);
+      put_snapshot<MODE>(A, dst, i, slot, x, y, z);
       break;
     }
     default:

@@ -110,8 +110,14 @@ size_t Engine::carve(char* base) {
   A_.py = D(kMaxSplit);
   A_.pz = D(kMaxSplit);
   A_.pstride = (int64_t)(dbytes / sizeof(double));
+  for (int b = 0; b < 2; ++b) {
+    A_.hx[b] = D();
+    A_.hy[b] = D();
+    A_.hz[b] = D();
+  }
   A_.perm = (int32_t*)F();
   A_.gm32 = F();
+  A_.hg = F();
   A_.cap = cap_;
   return used;
 }
@@ -486,9 +492,11 @@ void Engine::run_round(std::vector<WorkItem>& items,
 // never prepared (or whose snapshot a parameter or mode change invalidated).
 void Engine::prepare(const std::vector<int>& ids) {
   std::vector<WorkItem> items;
+  std::vector<int> fresh;
   for (int id : ids) {
     Sys& s = sys_[id];
     if (s.prepared) continue;
+    fresh.push_back(id);
     const int stage = (s.p.integ == INTEG_VV && s.vv_primed) ? STAGE_VV_PRIME
                                                              : STAGE_ACC;
     tiles_of(id, stage, s.cur, items);
@@ -504,6 +512,27 @@ void Engine::prepare(const std::vector<int>& ids) {
             "hipMemcpy(work list)");
   hip_check(launch_prepare(d_arena_, d_items_, (int)items.size(), d_sys_, stream_),
             "prepare launch");
+  // MODE_F32R reads sources from the slot-ordered copy: refresh it where the
+  // order is already set (otherwise the next step's sort builds it)
+  if (mode_ == MODE_F32R && gravity_ == GRAV_DIRECT) {
+    std::vector<SortSys> tab;
+    int max_tiles = 0;
+    for (int id : fresh) {
+      const Sys& s = sys_[id];
+      if (s.sorted_at < 0) continue;
+      SortSys e{};
+      e.off = s.off;
+      e.n = (int32_t)s.p.n;
+      tab.push_back(e);
+      max_tiles = std::max<int>(max_tiles, (int)((e.n + kWave - 1) / kWave));
+    }
+    if (!tab.empty()) {
+      upload_sort_tab(tab);
+      hip_check(launch_sort_scatter(d_arena_, d_sort_tab_, (int)tab.size(),
+                                    max_tiles, nullptr, stream_),
+                "snapshot copy launch");
+    }
+  }
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
@@ -1046,6 +1075,13 @@ std::vector<Engine::SortPass> Engine::plan_sorts(const std::vector<int>& ids,
   if (plan.empty()) return plan;
   for (size_t q = 0; q < ids.size(); ++q)
     if (last[q] >= 0) sys_[ids[q]].sorted_at = last[q];
+  upload_sort_tab(tab);
+  return plan;
+}
+
+// Copies a table of SortSys rows to d_sort_tab_ (grown as needed) and waits
+// for the copy (the rows are host memory of the caller).
+void Engine::upload_sort_tab(const std::vector<SortSys>& tab) {
   if (tab.size() > sort_tab_cap_) {
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     hipFree(d_sort_tab_);
@@ -1070,7 +1106,6 @@ std::vector<Engine::SortPass> Engine::plan_sorts(const std::vector<int>& ids,
                            hipMemcpyHostToDevice, stream_),
             "hipMemcpy(sort table)");
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  return plan;
 }
 
 void Engine::run_sorts(const std::vector<SortPass>& plan, size_t& next,

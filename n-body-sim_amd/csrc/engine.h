@@ -153,6 +153,7 @@ class Engine {
   };
   static constexpr int64_t kResortEvery = 32;
   std::vector<SortPass> plan_sorts(const std::vector<int>& ids, int64_t nsteps);
+  void upload_sort_tab(const std::vector<SortSys>& tab);
   void run_sorts(const std::vector<SortPass>& plan, size_t& next, int64_t t);
 
   size_t carve(char* base);

@@ -28,9 +28,12 @@ hipError_t launch_diag(const Arena* A, const SysDesc* sys, const int32_t* list,
 hipError_t launch_gather(const Arena* A, const WorkItem* items, int nitems,
                          const SysDesc* sys, double* stage, int64_t total,
                          int nfields, hipStream_t s);
-// MODE_F32C target order: keys (rank << 48 | Hilbert index) and local indices
+// MODE_F32R body order: keys (rank << 48 | Hilbert index) and local indices
 // of the listed systems' bodies, a radix sort (sort_pairs: hipCUB; query the
 // temporary size with tmp == nullptr), then perm from the sorted values.
+// launch_sort_scatter also rebuilds the slot-ordered snapshot copy (hx, hy,
+// hz, hg) from perm; with vals == nullptr it keeps perm and only rebuilds the
+// copy (after the body-order snapshot or g*m was rewritten).
 hipError_t launch_sort_keys(const Arena* A, const SortSys* tab, int nsys,
                             int max_tiles, uint64_t* keys, int32_t* vals,
                             hipStream_t s);

@@ -284,6 +284,9 @@ struct ChunkF32 {
 #ifndef NBODY_LDS_BATCH
 #define NBODY_LDS_BATCH 8
 #endif
+#ifndef NBODY_NEAR_K
+#define NBODY_NEAR_K 8.0f
+#endif
 constexpr int kLdsBatch = NBODY_LDS_BATCH;
 
 // v of lane L (an inactive lane holds its clamped body: any body will do).
@@ -294,61 +297,165 @@ __device__ __forceinline__ double lane_f64(double v, int L) {
   return __builtin_bit_cast(double, ((int64_t)hi << 32) | (uint32_t)lo);
 }
 
-// MODE_F32R sweep.  The wave's targets are 64 consecutive bodies along a
-// Hilbert curve (perm), so they sit close together; every coordinate is
-// taken relative to the wave's origin (its middle target, lane 32).  Each 64-source chunk is loaded once
-// per wave, one source per lane, differenced in f64 against the origin and
-// rounded once to f32, and staged in LDS; every lane then reads the sources
-// back as broadcasts, in batches, and the pair loop is pure f32.  The
-// rounding of a coordinate is relative to its distance from the origin:
-// small for the close pairs (near the wave's targets), relative to |dx| for
-// the far ones.  Group sums of 32 sources are added with Kahan compensation.
-// Sources past the system's end stage as zero mass.  NOMASK: eps2 > 0, so a
-// source at the target's own position (itself) contributes exactly zero;
-// otherwise pairs with r2 == 0 are masked (coincident bodies then add
-// nothing, where the reference's f64 path would produce NaN).  Faster than
-// MODE_F32C (no f64 work per pair), but a coordinate's rounding is relative
-// to its distance from the origin, so a close pair far from its wave's
-// origin loses accuracy (about 2^-24 * |x - origin| / |dx| of that term).
+// ---- MODE_F32R: relative coordinates, near/far chunks -------------------
+//
+// Targets and sources are both walked in Hilbert order (perm), so a wave's
+// 64 targets and each 64-source chunk are compact.  Every coordinate is taken
+// relative to the wave's origin o (its middle target, lane 32): s = f32(x - o)
+// from an f64 difference, plus the residual r = f32((x - o) - s).
+//
+// Far chunks use dx = s_j - s_i, a pure f32 pair loop.  Its error per
+// coordinate is at most 2^-24 (|x_i - o| + |x_j - o| + |dx|).  A chunk is far
+// when every source j in it has R + |s_j| <= K * D_j, where R bounds |s_i|
+// over the wave's targets and D_j is the distance from s_j to the targets'
+// bounding box (so D_j <= |dx| for every target).  Then each far pair's
+// difference is within (K + 1) 2^-24 |dx| of exact, whatever the shape of
+// the tile or where the pair sits.
+//
+// Near chunks (a source close to the tile compared with the distances to
+// the origin: the wave's own chunk, its neighbours along the curve, a tight
+// pair far from the origin) use dx = (s_j - s_i) + (r_j - r_i), which is
+// within about 2^-23 |dx| + 2^-48 (|x_i - o| + |x_j - o|) of exact, like
+// MODE_F32C's one rounding of an f64 difference.  The chunk's class is a
+// wave vote, so each chunk runs one loop.
+constexpr float kNearK = NBODY_NEAR_K;
+
+struct TargetR {
+  float x, y, z;        // s_i
+  float rx, ry, rz;     // r_i
+  float lox, loy, loz;  // bounding box of the wave's s_i
+  float hix, hiy, hiz;
+  float rad;            // max |s_i| over the wave
+};
+
+// Orders one wave's LDS accesses around a staging step.  Each wave stages
+// into its own LDS slice and a wave's LDS operations are performed in the
+// order issued, so no workgroup barrier is needed; this only stops the
+// compiler from moving memory accesses across the point.  (A workgroup
+// barrier would also wait for every outstanding global load, the prefetched
+// next chunk included.)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+
+__device__ __forceinline__ float uniform_f32(float v) {  // into an SGPR
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int32_t, v)));
+}
+
+// True unless the bound R + |s| <= K * D holds for source s (see above).
+// Overflow or NaN in the test makes the source near (the exact path).  The
+// test runs in f32 with v_sqrt_f32 (about 1 ulp); its rounding moves K by a
+// few parts in 10^7.
+__device__ __forceinline__ bool near_source(const TargetR& t, float sx,
+                                            float sy, float sz) {
+  const float ex = fmaxf(fmaxf(t.lox - sx, sx - t.hix), 0.f);
+  const float ey = fmaxf(fmaxf(t.loy - sy, sy - t.hiy), 0.f);
+  const float ez = fmaxf(fmaxf(t.loz - sz, sz - t.hiz), 0.f);
+  const float d2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+  const float a = t.rad + __builtin_amdgcn_sqrtf(fmaf(sz, sz, fmaf(sy, sy, sx * sx)));
+  return !(a * a <= (kNearK * kNearK) * d2);
+}
+
+// One staged chunk of 64 sources against the lane's target.  RES adds the
+// residual correction (near chunks).  Group sums of 32 sources are added with
+// Kahan compensation.  NOMASK: eps2 > 0, so a source at the target's own
+// position (itself) contributes exactly zero; otherwise pairs with r2 == 0
+// are masked (coincident bodies then add nothing, where the reference's f64
+// path would produce NaN).  Sources past the system's end stage as zero mass.
+template <bool NOMASK, bool RES>
+__device__ __forceinline__ void chunk_f32r(const float4* stage,
+                                           const float4* res,
+                                           const TargetR& t, float eps2,
+                                           Kahan3& acc) {
+  constexpr int B = RES ? kLdsBatch / 2 : kLdsBatch;
+#pragma unroll 1
+  for (int q = 0; q < kWave; q += 32) {
+    float px = 0.f, py = 0.f, pz = 0.f;
+#pragma unroll
+    for (int u0 = 0; u0 < 32; u0 += B) {
+      float4 s[B], r[B];  // issue the batch's reads before any use
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        s[u] = stage[q + u0 + u];
+        if (RES) r[u] = res[q + u0 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        float dx = s[u].x - t.x, dy = s[u].y - t.y, dz = s[u].z - t.z;
+        if (RES) {
+          dx = dx + (r[u].x - t.rx);
+          dy = dy + (r[u].y - t.ry);
+          dz = dz + (r[u].z - t.rz);
+        }
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        float k = s[u].w * ((inv * inv) * inv);
+        if (!NOMASK) k = r2 > 0.f ? k : 0.f;
+        px = fmaf(k, dx, px);
+        py = fmaf(k, dy, py);
+        pz = fmaf(k, dz, pz);
+      }
+    }
+    kahan_add(acc.sx, acc.cx, px);
+    kahan_add(acc.sy, acc.cy, py);
+    kahan_add(acc.sz, acc.cz, pz);
+  }
+}
+
+// Sources in slots [jlo, jhi) (64-aligned, Hilbert order), read from the
+// slot-ordered snapshot.  Each chunk is loaded once per wave, one source per
+// lane, differenced in f64 against the origin, rounded to s, voted near or
+// far, and staged in LDS (with the residuals r when near); every lane then
+// reads the sources back as broadcasts.  Sources past the system's end stage
+// as zero mass at the origin.
 template <bool NOMASK>
-__device__ __forceinline__ void sweep_f32_lds(
-    float4* stage, const double* X, const double* Y, const double* Z,
-    const float* G, int64_t jlo, int64_t jhi, int64_t end, double ox,
-    double oy, double oz, float xi, float yi, float zi, float eps2,
-    Kahan3& acc) {
+__device__ __forceinline__ void sweep_f32r(float4* stage, float4* res,
+                                           const double* X, const double* Y,
+                                           const double* Z, const float* G,
+                                           int64_t jlo, int64_t jhi,
+                                           int64_t end, double ox, double oy,
+                                           double oz, const TargetR& t,
+                                           float eps2, Kahan3& acc) {
   const int lane = (int)(threadIdx.x & (kWave - 1));
   for (int64_t j0 = jlo; j0 < jhi; j0 += kWave) {
     const int64_t j = j0 + lane;
-    float4 v = {0.f, 0.f, 0.f, 0.f};
-    if (j < end)
-      v = {(float)(X[j] - ox), (float)(Y[j] - oy), (float)(Z[j] - oz), G[j]};
-    stage[lane] = v;
-    __syncthreads();
-#pragma unroll 1
-    for (int q = 0; q < kWave; q += 32) {
-      float px = 0.f, py = 0.f, pz = 0.f;
-#pragma unroll
-      for (int u0 = 0; u0 < 32; u0 += kLdsBatch) {
-        float4 s[kLdsBatch];  // issue the batch's reads before any use
-#pragma unroll
-        for (int u = 0; u < kLdsBatch; ++u) s[u] = stage[q + u0 + u];
-#pragma unroll
-        for (int u = 0; u < kLdsBatch; ++u) {
-          const float dx = s[u].x - xi, dy = s[u].y - yi, dz = s[u].z - zi;
-          const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-          const float inv = __builtin_amdgcn_rsqf(r2);
-          float k = s[u].w * ((inv * inv) * inv);
-          if (!NOMASK) k = r2 > 0.f ? k : 0.f;
-          px = fmaf(k, dx, px);
-          py = fmaf(k, dy, py);
-          pz = fmaf(k, dz, pz);
-        }
-      }
-      kahan_add(acc.sx, acc.cx, px);
-      kahan_add(acc.sy, acc.cy, py);
-      kahan_add(acc.sz, acc.cz, pz);
+    const bool valid = j < end;
+    double dx = 0.0, dy = 0.0, dz = 0.0;
+    float g = 0.f;
+    if (valid) {
+      dx = X[j] - ox;
+      dy = Y[j] - oy;
+      dz = Z[j] - oz;
+      g = G[j];
     }
-    __syncthreads();
+    const float sx = (float)dx, sy = (float)dy, sz = (float)dz;
+    stage[lane] = {sx, sy, sz, g};
+    const bool near = valid && near_source(t, sx, sy, sz);
+    const bool any_near = __ballot(near) != 0;  // wave-uniform
+    if (any_near)
+      res[lane] = {(float)(dx - (double)sx), (float)(dy - (double)sy),
+                   (float)(dz - (double)sz), 0.f};
+    wave_sync();
+    if (any_near)
+      chunk_f32r<NOMASK, true>(stage, res, t, eps2, acc);
+    else
+      chunk_f32r<NOMASK, false>(stage, res, t, eps2, acc);
+    wave_sync();
   }
 }
 
@@ -377,25 +484,41 @@ __global__ __launch_bounds__(kBlock) void k_force(const Arena* Ap,
 
   const int nsplit = sd->nsplit, part = it.part;
   if (MODE == MODE_F32R) {
-    __shared__ float4 stage[kWavesPerBlock][kWave];
-    const double ox = lane_f64(A->sx[src][il], 32);  // the middle target
-    const double oy = lane_f64(A->sy[src][il], 32);
-    const double oz = lane_f64(A->sz[src][il], 32);
-    const float xi = (float)(A->sx[src][il] - ox);
-    const float yi = (float)(A->sy[src][il] - oy);
-    const float zi = (float)(A->sz[src][il] - oz);
+    __shared__ float4 stage[kWavesPerBlock][2][kWave];  // staged s, residuals
+    const int64_t ts = active ? slot : end - 1;  // in this tile
+    const double px0 = A->hx[src][ts], py0 = A->hy[src][ts],
+                 pz0 = A->hz[src][ts];
+    const double ox = lane_f64(px0, 32);  // the middle target
+    const double oy = lane_f64(py0, 32);
+    const double oz = lane_f64(pz0, 32);
+    const double tx = px0 - ox, ty = py0 - oy, tz = pz0 - oz;
+    TargetR t;
+    t.x = (float)tx;
+    t.y = (float)ty;
+    t.z = (float)tz;
+    t.rx = (float)(tx - (double)t.x);
+    t.ry = (float)(ty - (double)t.y);
+    t.rz = (float)(tz - (double)t.z);
+    t.lox = uniform_f32(wave_min(t.x));
+    t.loy = uniform_f32(wave_min(t.y));
+    t.loz = uniform_f32(wave_min(t.z));
+    t.hix = uniform_f32(wave_max(t.x));
+    t.hiy = uniform_f32(wave_max(t.y));
+    t.hiz = uniform_f32(wave_max(t.z));
+    t.rad = uniform_f32(wave_max(
+        __builtin_amdgcn_sqrtf(fmaf(t.z, t.z, fmaf(t.y, t.y, t.x * t.x)))));
     Kahan3 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     int64_t jlo, jhi;
     part_range<kWave>(off, n, nsplit, part, jlo, jhi);
     const float e2 = sd->eps2f;
     if (e2 > 0.f)
-      sweep_f32_lds<true>(stage[wave], A->sx[src], A->sy[src], A->sz[src],
-                          A->gm32, jlo, jhi, end, ox, oy, oz, xi, yi, zi, e2,
-                          acc);
+      sweep_f32r<true>(stage[wave][0], stage[wave][1], A->hx[src], A->hy[src],
+                       A->hz[src], A->hg, jlo, jhi, end, ox, oy, oz, t, e2,
+                       acc);
     else
-      sweep_f32_lds<false>(stage[wave], A->sx[src], A->sy[src], A->sz[src],
-                           A->gm32, jlo, jhi, end, ox, oy, oz, xi, yi, zi, e2,
-                           acc);
+      sweep_f32r<false>(stage[wave][0], stage[wave][1], A->hx[src], A->hy[src],
+                        A->hz[src], A->hg, jlo, jhi, end, ox, oy, oz, t, e2,
+                        acc);
     ax = (double)acc.sx - (double)acc.cx;
     ay = (double)acc.sy - (double)acc.cy;
     az = (double)acc.sz - (double)acc.cz;
@@ -445,7 +568,9 @@ __global__ __launch_bounds__(kBlock) void k_force(const Arena* Ap,
   }
   if (!active) return;
   if (sd->nsplit > 1) {  // partial sum; k_combine finishes the target
-    const int64_t q = (int64_t)it.part * A->pstride + i;
+    // MODE_F32R keeps partials in slot order (k_combine walks slots)
+    const int64_t q =
+        (int64_t)it.part * A->pstride + (MODE == MODE_F32R && !POT ? slot : i);
     A->px[q] = ax;
     if (!POT) {
       A->py[q] = ay;
@@ -457,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const Arena* Ap,
     A->phi[i] = ax;
     return;
   }
-  epilogue<MODE>(A, sd, it.stage, src ^ 1, i, ax, ay, az);
+  epilogue<MODE>(A, sd, it.stage, src ^ 1, i, slot, ax, ay, az);
 }
 
 // Adds the parts' partial sums of each target in part order (a fixed order,
@@ -474,23 +599,26 @@ __global__ __launch_bounds__(kBlock) void k_combine(const Arena* Ap,
   const CArena* A = (const CArena*)Ap;
   const CSys* sd = (const CSys*)(sysd + it.sys);
   const int64_t end = sd->off + sd->n;
-  const int64_t i = sd->off + (int64_t)it.tile * kWave +
+  // the tile's targets: bodies, or in MODE_F32R slots in Hilbert order (the
+  // potential pass, computed in f64, is always in body order)
+  const int64_t k = sd->off + (int64_t)it.tile * kWave +
                     (int64_t)(threadIdx.x & (kWave - 1));
-  if (i >= end) return;
+  if (k >= end) return;
   const int64_t st = A->pstride;
-  double ax = A->px[i];
+  double ax = A->px[k];
   if (it.stage == STAGE_POT) {
-    for (int p = 1; p < sd->nsplit; ++p) ax = ax + A->px[p * st + i];
-    A->phi[i] = ax;
+    for (int p = 1; p < sd->nsplit; ++p) ax = ax + A->px[p * st + k];
+    A->phi[k] = ax;
     return;
   }
-  double ay = A->py[i], az = A->pz[i];
+  double ay = A->py[k], az = A->pz[k];
   for (int p = 1; p < sd->nsplit; ++p) {
-    ax = ax + A->px[p * st + i];
-    ay = ay + A->py[p * st + i];
-    az = az + A->pz[p * st + i];
+    ax = ax + A->px[p * st + k];
+    ay = ay + A->py[p * st + k];
+    az = az + A->pz[p * st + k];
   }
-  epilogue<MODE>(A, sd, it.stage, it.src ^ 1, i, ax, ay, az);
+  const int64_t i = MODE == MODE_F32R ? sd->off + A->perm[k] : k;
+  epilogue<MODE>(A, sd, it.stage, it.src ^ 1, i, k, ax, ay, az);
 }
 
 // Builds source snapshot it.src of each body from the master state:
@@ -695,7 +823,23 @@ __global__ __launch_bounds__(kWave) void k_sort_scatter(const Arena* Ap,
   const int k = (int)blockIdx.x * kWave + (int)threadIdx.x;
   if (k >= t.n) return;
   const CArena* A = (const CArena*)Ap;
-  A->perm[t.off + k] = vals[t.base + k];
+  const int64_t q = t.off + k;
+  int32_t v;
+  if (vals) {
+    v = vals[t.base + k];
+    A->perm[q] = v;
+  } else {
+    v = A->perm[q];
+  }
+  // the slot-ordered copy of both snapshot buffers (only the current one
+  // matters; the next force round rewrites the other) and of g*m
+  const int64_t i = t.off + v;
+  for (int b = 0; b < 2; ++b) {
+    A->hx[b][q] = A->sx[b][i];
+    A->hy[b][q] = A->sy[b][i];
+    A->hz[b][q] = A->sz[b][i];
+  }
+  A->hg[q] = A->gm32[i];
 }
 
 }  // namespace

@@ -4,7 +4,9 @@
 //   m, gm64 (= G*m), gm32         bodies' masses, G folded in once per system
 //   x, y, z, vx, vy, vz           master state, binary64
 //   sx/sy/sz[2]                   f64 source snapshot, double-buffered
-//   perm                          MODE_F32R target order (Hilbert curve)
+//   perm                          MODE_F32R body order (Hilbert curve)
+//   hx/hy/hz[2], hg               MODE_F32R: the snapshot and g*m in that
+//                                 order, so sources read coalesced
 //   ax, ay, az                    carried acceleration (velocity Verlet) or the
 //                                 output of a plain force evaluation
 //   rk[9]                         RK4 running sums and next-stage velocity
@@ -35,10 +37,11 @@ enum Mode : int32_t {            // force arithmetic (public NBODY_PRECISION_*)
   MODE_F32C = 2,                 // f64 positions, each difference rounded
                                  // once to f32, f32 pair terms, compensated
                                  // accumulation
-  MODE_F32R = 3,                 // targets in Hilbert order, coordinates
+  MODE_F32R = 3,                 // bodies in Hilbert order, coordinates
                                  // relative to each wave's middle target
-                                 // rounded to f32, sources staged in LDS,
-                                 // f32 pair terms, compensated accumulation
+                                 // (f32 value + f32 residual), sources staged
+                                 // in LDS, f32 pair terms (residuals added
+                                 // for near chunks), compensated accumulation
 };
 
 enum Integ : int32_t { INTEG_EULER = 0, INTEG_VV = 1, INTEG_RK4 = 2 };
@@ -91,8 +94,10 @@ struct Arena {         // device pointers, passed by value to every kernel
   double *rk[9];
   double *phi;
   double *px, *py, *pz; // per-part partial sums: [part * pstride + i]
-  int32_t* perm;        // MODE_F32R target order: tile slot off + k holds
-                        // body off + perm[off + k] (Hilbert order)
+  int32_t* perm;        // MODE_F32R order: slot off + k holds body
+                        // off + perm[off + k] (Hilbert order)
+  double *hx[2], *hy[2], *hz[2];  // MODE_F32R: sx/sy/sz in slot order
+  float* hg;            // MODE_F32R: gm32 in slot order
   int64_t pstride;     // cap + kPad
   int64_t cap;         // bodies (excluding the kPad slack)
 };

@@ -154,7 +154,7 @@ class Precision(enum.IntEnum):
     F64_STRICT = 0   # reference operation order: bit-identical to the f64 CPU path
     F64 = 1          # f64, rsq + refinement, FMA
     F32C = 2         # f64 differences rounded to f32, f32 terms, compensated sums
-    F32R = 3         # f32 relative to each wave's middle target, Hilbert order (faster)
+    F32R = 3         # f32 relative to each wave's middle target, Hilbert order, near chunks corrected (fastest for large systems)
 
 
 class GravityMethod(enum.IntEnum):

@@ -1,0 +1,121 @@
+"""CPU model of MODE_F32R's near/far chunk rule (n-body-sim_amd/csrc/kernels.hip,
+near_source / sweep_f32r / chunk_f32r).
+
+Targets and sources run in Hilbert order; each wave of 64 targets takes its
+middle target as origin o.  A coordinate becomes s = f32(x - o) plus the
+residual r = f32((x - o) - s).  A 64-source chunk is "far" when every source
+in it satisfies R + |s_j| <= K * D_j (R: max |s_i| over the wave; D_j: the
+distance from s_j to the box of the wave's s_i), and then uses dx = s_j - s_i;
+otherwise dx = (s_j - s_i) + (r_j - r_i).
+
+This replays that arithmetic in numpy float32 (one rounding per operation,
+as on the device without contraction) and checks the two claims the kernel's
+accuracy rests on, on a geometry built to break the plain relative form:
+tight pairs in a sparse halo, far from their wave's origin.
+  * every far pair's difference is within (K + 1) 2^-24 |dx| of exact;
+  * the resulting accelerations stay within 1e-5 of the oracle's f64 sum,
+    while the relative form without the near path does not.
+"""
+import numpy as np
+
+import oracle
+from test_kernel_bounds import hilbert48
+
+K = 8.0        # NBODY_NEAR_K
+U = 2.0 ** -24
+F = np.float32
+
+
+def hilbert_order(x, y, z, m):
+    """The device's target order: quant16 around the centre of mass in a box
+    of width 4 * extent, hilbert48, stable sort."""
+    com = [np.sum(m * v) / m.sum() for v in (x, y, z)]
+    ext = max(np.abs(v - c).max() for v, c in zip((x, y, z), com))
+    inv = 1.0 / (4.0 * ext)
+
+    def q(v, o):
+        return (np.clip((v - o) * inv + 0.5, 0.0, 0.99999) * 65536.0).astype(np.int64)
+
+    qx, qy, qz = q(x, com[0]), q(y, com[1]), q(z, com[2])
+    keys = [hilbert48(int(a), int(b), int(c)) for a, b, c in zip(qx, qy, qz)]
+    return np.argsort(np.array(keys, dtype=np.uint64), kind="stable")
+
+
+def halo_with_tight_pairs(seed=3):
+    """1536 Plummer bodies plus 256 tight pairs (separation 1e-4 of their
+    radius) at radii 3..8: close pairs far from any wave's origin."""
+    rng = np.random.default_rng(seed)
+    n0, npair = 1536, 256
+    p = rng.standard_normal((n0, 3))
+    p *= (rng.uniform(0.02, 1.5, n0) / np.linalg.norm(p, axis=1))[:, None]
+    d = rng.standard_normal((npair, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    c = d * rng.uniform(3.0, 8.0, npair)[:, None]
+    e = rng.standard_normal((npair, 3))
+    e /= np.linalg.norm(e, axis=1)[:, None]
+    sep = 1e-4 * np.linalg.norm(c, axis=1)[:, None]
+    pos = np.concatenate([p, c, c + sep * e])
+    m = np.full(len(pos), 1.0 / len(pos))
+    return m, pos[:, 0].copy(), pos[:, 1].copy(), pos[:, 2].copy()
+
+
+def model_accelerations(m, P, eps2, near_path=True):
+    """Accelerations of every body (slot order) as MODE_F32R forms them,
+    with per-pair terms summed in f64 (the model checks the differences,
+    not the kernel's compensated f32 summation).  Also returns the worst
+    far-pair difference error in units of 2^-24 |dx|."""
+    n = len(m)
+    g = m.astype(F)
+    acc = np.zeros((n, 3))
+    worst = 0.0
+    for t0 in range(0, n, 64):
+        T = P[t0:t0 + 64]
+        o = T[min(32, len(T) - 1)]
+        S = (P - o).astype(F)                        # s for every body
+        R = ((P - o) - S.astype(np.float64)).astype(F)   # residuals
+        si, ri = S[t0:t0 + 64], R[t0:t0 + 64]
+        lo, hi = si.min(0), si.max(0)
+        rad = np.sqrt((si * si).sum(1, dtype=F)).max()
+        ev = np.maximum(np.maximum(lo - S, S - hi), F(0))
+        D2 = (ev * ev).sum(1, dtype=F)
+        a = rad + np.sqrt((S * S).sum(1, dtype=F))
+        near_src = ~(a * a <= F(K * K) * D2)
+        near_chunk = np.zeros((n + 63) // 64, bool)
+        np.logical_or.at(near_chunk, np.arange(n) // 64, near_src)
+        near = near_chunk[np.arange(n) // 64] & near_path
+        dx = S[None, :, :] - si[:, None, :]                      # far form
+        dn = dx + (R[None, :, :] - ri[:, None, :])              # near form
+        exact = P[None, :, :] - T[:, None, :]
+        far = ~near
+        if far.any():
+            err = np.abs(dx[:, far, :].astype(np.float64) - exact[:, far, :])
+            nrm = np.linalg.norm(exact[:, far, :], axis=2)
+            ok = nrm > 0
+            if ok.any():
+                worst = max(worst, (np.linalg.norm(err, axis=2)[ok] / nrm[ok]).max() / U)
+        d = np.where(near[None, :, None], dn, dx)
+        r2 = (d * d).sum(2, dtype=F) + F(eps2)
+        k = g[None, :] / (r2 * np.sqrt(r2))
+        k[np.arange(len(T)), np.arange(t0, t0 + len(T))] = 0
+        acc[t0:t0 + 64] = np.einsum("ij,ijk->ik", k.astype(np.float64),
+                                    d.astype(np.float64))
+    return acc, worst
+
+
+def test_far_pairs_within_bound_and_accuracy():
+    m, x, y, z = halo_with_tight_pairs()
+    perm = hilbert_order(x, y, z, m)
+    P = np.stack([x, y, z], 1)[perm]
+    mp = m[perm]
+    eps = 1e-6
+    ref = np.stack(oracle.newton_parallel(mp, P[:, 0].copy(), P[:, 1].copy(),
+                                          P[:, 2].copy(), 1.0, eps), 1)
+    acc, worst = model_accelerations(mp, P, eps * eps)
+    # one rounding of each relative coordinate and of the f32 subtraction
+    assert worst <= (K + 1) * 1.001, worst
+    e = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert e.max() <= 1e-5, e.max()
+    # the relative form alone (no near path) fails on the tight pairs
+    acc0, _ = model_accelerations(mp, P, eps * eps, near_path=False)
+    e0 = np.linalg.norm(acc0 - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert e0.max() > 1e-4, e0.max()

@@ -124,9 +124,41 @@ def test_config2_size_f64_and_f32c(gpu):
           f"f32r max {e32r.max():.3e} med {np.median(e32r):.3e}", flush=True)
     assert e64.max() <= 1e-12
     assert e32.max() <= 1e-5
-    # the re-centred f32 snapshot form: rounding of absolute positions limits
-    # close pairs away from the centre (measured 5.4e-5 max here)
-    assert e32r.max() <= 5e-5 and np.median(e32r) <= 1e-7
+    assert e32r.max() <= 1e-5 and np.median(e32r) <= 1e-7
+
+
+@pytest.mark.parametrize("scenario", ["star", "colliding", "disc"])
+def test_f32_scenarios_within_1e5(gpu, scenario):
+    """The f32 forms on the other workloads' geometries: a central mass with
+    a thin population of light bodies, two separated spheres, a thin disc."""
+    if scenario == "star":
+        b, eps = nb.star_system(6000, 11, m_lo=1e-7, m_hi=1e-4, r_lo=0.5,
+                                r_hi=10.0, max_incl_deg=5.0), 1e-3
+    elif scenario == "colliding":
+        b, eps = nb.colliding(8192, 12, separation=10.0, rel_speed=0.5), 0.002
+    else:
+        b, eps = nb.disc_galaxy(8192, 13), 0.05
+    ref = oracle.newton_parallel(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, eps)
+    for prec in (nb.Precision.F32R, nb.Precision.F32C):
+        e = rel_err(gpu_acc(gpu, b, 1.0, eps, prec), ref)
+        print(f"{scenario} {prec.name} max {e.max():.3e} med {np.median(e):.3e}",
+              flush=True)
+        assert e.max() <= 1e-5, prec
+
+
+@pytest.mark.parametrize("prec", [nb.Precision.F32R, nb.Precision.F32C])
+def test_f32_tight_pairs_far_from_origin(gpu, prec):
+    """Tight pairs in a sparse halo, far from their wave's origin: the case
+    MODE_F32R's near chunks exist for (the relative f32 form alone loses
+    about 1e-3 here; see test_f32r_model.py)."""
+    from test_f32r_model import halo_with_tight_pairs
+    m, x, y, z = halo_with_tight_pairs()
+    zero = np.zeros_like(m)
+    b = nb.Bodies(m, x, y, z, zero, zero, zero)
+    ref = oracle.newton_parallel(m, x, y, z, 1.0, 1e-6)
+    e = rel_err(gpu_acc(gpu, b, 1.0, 1e-6, prec), ref)
+    print(f"{prec.name} max {e.max():.3e} med {np.median(e):.3e}", flush=True)
+    assert e.max() <= 1e-5 and np.median(e) <= 1e-7
 
 
 @pytest.mark.parametrize("integ", INTEG)
