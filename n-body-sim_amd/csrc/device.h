@@ -46,14 +46,15 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
 
 // x^(-3/2) in binary64 with the refinement folded into the cube: with the
 // v_rsq_f64 seed y and e = 1 - x*y^2 (|e| about 2^-22),
-//   (y (1 - e)^(-1/2))^3 = y^3 (1 - e)^(-3/2) = y^3 (1 + 1.5 e + O(e^2)),
-// so one fused multiply-add on t = y^3 replaces refining y before cubing
-// (the dropped e^2 term is below 2^-43 relative).
+//   (y (1 - e)^(-1/2))^3 = y^3 (1 - e)^(-3/2) = y^3 (1 + 1.5 e + O(e^2))
+//                        = y^3 (2.5 - 1.5 x y^2) + O(e^2),
+// so the cube takes the correction as one factor instead of refining y
+// first (the dropped e^2 term is below 2^-43 relative).
 __device__ __forceinline__ double inv_cube_f64(double x) {
   const double y = __builtin_amdgcn_rsq(x);
-  const double e = fma(-x * y, y, 1.0);
-  const double t = (y * y) * y;
-  return fma(t, 1.5 * e, t);
+  const double u = y * y;
+  const double c = fma(-1.5, x * u, 2.5);
+  return (y * u) * c;
 }
 
 // Writes the next source snapshot of body i (f64 positions, every mode);

@@ -3,8 +3,8 @@
 For every bucket b (systems of n with floor(log2 n) == b, sampled at
 n = 2^b and n = 1.5 * 2^b), each candidate part count k in {1, 2, 4, 8, 16, 32} (with
 at least 64 sources per part) is timed on a Plummer sphere stepped with
-velocity Verlet in the f32-compensated and the f64 arithmetic.  The k with
-the smallest summed time wins.  The table is written as
+velocity Verlet in both f32-compensated forms (f32r, f32c) and the f64
+arithmetic.  The k with the smallest summed time wins.  The table is written as
 `split <bucket> <parts>` lines (the format the library loads at start-up
 from tuning/gfx950.txt next to libnbody_amd.so).
 
@@ -24,6 +24,7 @@ if PKG not in sys.path:
 import nbody_amd as nb  # noqa: E402
 
 CANDIDATES = (1, 2, 4, 8, 16, 32)
+TUNED = (nb.Precision.F32R, nb.Precision.F32C, nb.Precision.F64)
 
 
 def time_case(gpu, bodies, prec, budget_ms=40.0):
@@ -64,7 +65,7 @@ def tune(max_bucket=20, min_bucket=6, log=print):
             nb.set_tuning_split(trial)
             t = 0.0
             for bodies in systems:
-                for prec in (nb.Precision.F32C, nb.Precision.F64):
+                for prec in TUNED:
                     t += time_case(gpu, bodies, prec)
             row.append((k, t))
             if best_t is None or t < best_t:
@@ -81,7 +82,7 @@ def tune(max_bucket=20, min_bucket=6, log=print):
 def write_table(path, table, report, max_bucket):
     lines = ["# source-range parts per target tile by bucket floor(log2 n)",
              "# written by nbody_amd/autotune.py on " + time.strftime("%Y-%m-%d"),
-             "# sum of ms/step (f32c + f64, velocity Verlet, Plummer) per candidate:"]
+             "# sum of ms/step (f32r + f32c + f64, velocity Verlet, Plummer) per candidate:"]
     for b, sizes, row, best in report:
         lines.append("#   bucket %d n=%s: %s -> %d" % (
             b, sizes, " ".join(f"{k}:{t:.3f}" for k, t in row), best))
