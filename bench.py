@@ -511,8 +511,9 @@ def main():
     ap.add_argument("--workload",
                     choices=["single", "batched", "serve", "large", "bh", "copy"],
                     default="single")
-    ap.add_argument("--rates", default="20,50,80", help="serve: request rates (1/s)")
-    ap.add_argument("--requests", type=int, default=150, help="serve: requests per rate")
+    ap.add_argument("--rates", default="20,50,80,160,640",
+                    help="serve: request rates (1/s); the last ones saturate the GPU")
+    ap.add_argument("--requests", type=int, default=200, help="serve: requests per rate")
     ap.add_argument("--mix", default="plummer,star",
                     help="serve: request scenarios, from plummer, star, disc")
     ap.add_argument("--f32", choices=sorted(F32_FORMS), default=None,

@@ -55,8 +55,10 @@ Engine::~Engine() {
   hipStreamSynchronize(stream_);
   hipFree(d_items_);
   hipFree(d_step_items_);
-  hipFree(d_stage_);
-  hipHostFree(h_stage_);
+  for (Staging& st : stage_) {
+    hipFree(st.d);
+    hipHostFree(st.h);
+  }
   hipFree(d_traj_);
   hipFree(d_snap_items_);
   hipFree(d_sort_tab_);
@@ -142,17 +144,25 @@ void Engine::alloc_arena() {
 // Host-side guard run on every work list before it is uploaded: each item
 // must name a live system, a tile inside it and a snapshot buffer 0 or 1.
 void Engine::validate_items(const std::vector<WorkItem>& items) const {
+  // items come in runs of one system: its limits are looked up per run
+  int cur = -1;
+  int64_t tiles = 0;
+  int parts = 1;
   for (const WorkItem& it : items) {
-    if (it.sys < 0 || it.sys >= max_sys_ || !sys_[it.sys].live)
-      throw Error(NBODY_E_STATE, "work item names no live system");
-    const Sys& s = sys_[it.sys];
-    const int64_t tiles = (s.p.n + kWave - 1) / kWave;
+    if (it.sys != cur) {
+      if (it.sys < 0 || it.sys >= max_sys_ || !sys_[it.sys].live)
+        throw Error(NBODY_E_STATE, "work item names no live system");
+      const Sys& s = sys_[it.sys];
+      if (s.off < 0 || s.off + s.span > cap_ || s.span < s.p.n)
+        throw Error(NBODY_E_STATE, "system range outside the arena");
+      cur = it.sys;
+      tiles = (s.p.n + kWave - 1) / kWave;
+      parts = std::max(1, split_of(it.sys));
+    }
     if (it.tile < 0 || it.tile >= tiles || (it.src & ~1) != 0 ||
         it.stage < STAGE_ACC || it.stage > STAGE_POT || it.part < 0 ||
-        it.part >= std::max(1, split_of(it.sys)))
+        it.part >= parts)
       throw Error(NBODY_E_STATE, "work item out of range");
-    if (s.off < 0 || s.off + s.span > cap_ || s.span < s.p.n)
-      throw Error(NBODY_E_STATE, "system range outside the arena");
   }
 }
 
@@ -544,10 +554,12 @@ void Engine::prepare(const std::vector<int>& ids) {
 void Engine::step_lists(const std::vector<int>& ids, int flip,
                         std::vector<WorkItem> (&force)[4],
                         std::vector<WorkItem> (&combine)[4]) const {
+  // longest systems first (the order sort_items gives), built in that order
+  const std::vector<int> order = by_size(ids);
   for (int r = 0; r < 4; ++r) {
     force[r].clear();
     combine[r].clear();
-    for (int id : ids) {
+    for (int id : order) {
       const Sys& s = sys_[id];
       if (s.p.integ == INTEG_RK4) {
         tiles_of(id, STAGE_RK0 + r, s.cur ^ (r & 1), force[r], &combine[r]);
@@ -556,8 +568,15 @@ void Engine::step_lists(const std::vector<int>& ids, int flip,
                  s.cur ^ flip, force[r], &combine[r]);
       }
     }
-    sort_items(force[r]);
   }
+}
+
+std::vector<int> Engine::by_size(const std::vector<int>& ids) const {
+  std::vector<int> order(ids);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return sys_[a].p.n > sys_[b].p.n;
+  });
+  return order;
 }
 
 void Engine::sort_items(std::vector<WorkItem>& items) const {
@@ -735,17 +754,18 @@ void Engine::run_steps(const std::vector<int>& ids, int64_t nsteps,
   finish_steps(ids, nsteps);
 }
 
-void Engine::ensure_stage(size_t need) {
-  if (need <= stage_cap_) return;
+void Engine::ensure_stage(int which, size_t need) {
+  Staging& st = stage_[which];
+  if (need <= st.cap) return;
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  hipFree(d_stage_);
-  hipHostFree(h_stage_);
-  d_stage_ = nullptr;
-  h_stage_ = nullptr;
-  stage_cap_ = std::max(need, 2 * stage_cap_);
-  hip_check(hipMalloc((void**)&d_stage_, sizeof(double) * stage_cap_),
+  hipFree(st.d);
+  hipHostFree(st.h);
+  st.d = nullptr;
+  st.h = nullptr;
+  st.cap = std::max(need, 2 * st.cap);
+  hip_check(hipMalloc((void**)&st.d, sizeof(double) * st.cap),
             "hipMalloc(gather staging)");
-  hip_check(hipHostMalloc((void**)&h_stage_, sizeof(double) * stage_cap_,
+  hip_check(hipHostMalloc((void**)&st.h, sizeof(double) * st.cap,
                           hipHostMallocDefault),
             "hipHostMalloc(gather staging)");
 }
@@ -754,9 +774,26 @@ const double* Engine::step_snapshots(const std::vector<int>& ids,
                                      int64_t nsteps, bool graph,
                                      const std::vector<Snap>& snaps,
                                      std::vector<int64_t>& rows,
-                                     int64_t& total) {
+                                     int64_t& total,
+                                     const std::vector<int64_t>* counts,
+                                     const std::function<bool(int64_t, const double*)>& boundary,
+                                     int64_t* ran) {
   rows.clear();
   total = 0;
+  if (ran) *ran = nsteps;
+  if (counts) {
+    if (counts->size() != ids.size())
+      throw Error(NBODY_E_INVALID, "one step count per system");
+    for (int64_t c : *counts)
+      if (c < 0 || c > nsteps)
+        throw Error(NBODY_E_INVALID, "step count outside the call");
+  }
+  auto count_of = [&](int id) {
+    if (!counts) return nsteps;
+    for (size_t q = 0; q < ids.size(); ++q)
+      if (ids[q] == id) return (*counts)[q];
+    return (int64_t)-1;
+  };
   // gather lists, grouped by step: [first[t], first[t+1]) are step t's items
   std::vector<Snap> order(snaps);
   std::stable_sort(order.begin(), order.end(),
@@ -766,7 +803,7 @@ const double* Engine::step_snapshots(const std::vector<int>& ids,
   std::vector<int64_t> row_of(snaps.size());
   for (const Snap& sn : order) {
     check_id(sn.id);
-    if (sn.t < 0 || sn.t >= nsteps)
+    if (sn.t < 0 || sn.t >= nsteps || (counts && sn.t >= count_of(sn.id)))
       throw Error(NBODY_E_INVALID, "snapshot step outside the call");
   }
   for (size_t k = 0; k < order.size(); ++k) {
@@ -784,6 +821,7 @@ const double* Engine::step_snapshots(const std::vector<int>& ids,
     rows.push_back(total);
     total += n;
   }
+  const std::vector<int64_t> srow(rows);  // rows in step order
   // rows in the caller's order
   {
     std::vector<int64_t> r(snaps.size());
@@ -798,7 +836,7 @@ const double* Engine::step_snapshots(const std::vector<int>& ids,
   if (total > INT32_MAX) throw Error(NBODY_E_INVALID, "snapshots too large");
   if (!items.empty()) {
     validate_items(items);
-    ensure_stage((size_t)total * 3);
+    ensure_stage(0, (size_t)total * 3);
     if (items.size() > snap_items_cap_) {
       hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
       hipFree(d_snap_items_);
@@ -814,21 +852,63 @@ const double* Engine::step_snapshots(const std::vector<int>& ids,
               "hipMemcpy(snapshot lists)");
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   }
+  // rows [0, on_host) of every coordinate are on the host (step order)
+  int64_t on_host = 0;
+  auto to_host_before = [&](int64_t t) {  // snapshots taken before step t
+    size_t k = 0;
+    while (k < order.size() && order[k].t < t) ++k;
+    const int64_t end = k < order.size() ? srow[k] : total;
+    if (end <= on_host) return;
+    for (int f = 0; f < 3; ++f)
+      hip_check(hipMemcpyAsync(stage_[0].h + f * total + on_host,
+                               stage_[0].d + f * total + on_host,
+                               sizeof(double) * (end - on_host),
+                               hipMemcpyDeviceToHost, stream_),
+                "hipMemcpy(snapshots)");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    on_host = end;
+  };
+  int64_t stop = nsteps;
   size_t next = 0;
-  run_steps(ids, nsteps, graph, [&](int64_t t) {
+  auto gather_after = [&](int64_t t) {
     for (; next < at.size() && at[next].first == t; ++next) {
       const int lo = at[next].second.first, hi = at[next].second.second;
       hip_check(launch_gather(d_arena_, d_snap_items_ + lo, hi - lo, d_sys_,
-                              d_stage_, total, 3, stream_),
+                              stage_[0].d, total, 3, stream_),
                 "gather launch");
     }
-  });
+  };
+  if (!counts) {
+    run_steps(ids, nsteps, graph, gather_after);
+  } else {
+    // segments [t0, e) between the distinct counts, each over the systems
+    // whose count reaches e
+    std::vector<int64_t> ends(counts->begin(), counts->end());
+    std::sort(ends.begin(), ends.end());
+    ends.erase(std::unique(ends.begin(), ends.end()), ends.end());
+    const bool uniform = ends.size() == 1;
+    int64_t t0 = 0;
+    for (int64_t e : ends) {
+      if (e <= t0) continue;
+      std::vector<int> active;
+      for (size_t q = 0; q < ids.size(); ++q)
+        if ((*counts)[q] >= e) active.push_back(ids[q]);
+      run_steps(active, e - t0, graph && uniform,
+                [&](int64_t t) { gather_after(t0 + t); });
+      t0 = e;
+      if (t0 < nsteps && boundary) {
+        to_host_before(t0);
+        if (boundary(t0, total ? stage_[0].h : nullptr)) {
+          stop = t0;
+          break;
+        }
+      }
+    }
+  }
+  if (ran) *ran = stop;
   if (total == 0) return nullptr;
-  hip_check(hipMemcpyAsync(h_stage_, d_stage_, sizeof(double) * 3 * total,
-                           hipMemcpyDeviceToHost, stream_),
-            "hipMemcpy(snapshots)");
-  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  return h_stage_;
+  to_host_before(stop);
+  return stage_[0].h;
 }
 
 void Engine::accelerations(int id, double* ax, double* ay, double* az) {
@@ -939,7 +1019,7 @@ const double* Engine::gather(const std::vector<int>& ids, int nfields,
   if (total == 0) return nullptr;
   if (total > INT32_MAX) throw Error(NBODY_E_INVALID, "gather too large");
   const size_t need = (size_t)total * nfields;
-  ensure_stage(need);
+  ensure_stage(1, need);
   validate_items(items);
   ensure_items(items.size());
   hip_check(hipMemcpyAsync(d_items_, items.data(), sizeof(WorkItem) * items.size(),
@@ -947,13 +1027,13 @@ const double* Engine::gather(const std::vector<int>& ids, int nfields,
             "hipMemcpy(work list)");
   upload_table();
   hip_check(launch_gather(d_arena_, d_items_, (int)items.size(), d_sys_,
-                          d_stage_, total, nfields, stream_),
+                          stage_[1].d, total, nfields, stream_),
             "gather launch");
-  hip_check(hipMemcpyAsync(h_stage_, d_stage_, sizeof(double) * need,
+  hip_check(hipMemcpyAsync(stage_[1].h, stage_[1].d, sizeof(double) * need,
                            hipMemcpyDeviceToHost, stream_),
             "hipMemcpy(gather)");
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  return h_stage_;
+  return stage_[1].h;
 }
 
 void Engine::synchronize() {

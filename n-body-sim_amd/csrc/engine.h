@@ -11,6 +11,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -89,12 +90,25 @@ class Engine {
   // step() (graph = false) or step_graph() that also gathers the listed
   // snapshots on the device as the steps run -- a gather launch after each
   // step that has some -- and copies them to pinned host memory in one
-  // transfer at the end.  Returns a host view valid until the next gather:
+  // transfer at the end.  Returns a host view valid until the next call:
   // coordinate f of snapshot k starts at base[f * total + rows[k]].
+  // With `counts` (one per id, each in [0, nsteps]) system ids[q] takes only
+  // counts[q] of the call's steps: the call runs in segments between the
+  // distinct counts, each over the systems still stepping, so a short system
+  // drops out without ending the call (graphs are used only when every
+  // count is nsteps).  `boundary`, when given, is called at each boundary
+  // t0 < nsteps between segments, after the systems with count t0 have
+  // taken their last step, with the host view already holding every
+  // snapshot taken before step t0 (gather() may be called from it); when it
+  // returns true the call stops there: *ran receives the steps run (systems
+  // took min(count, *ran)) and snapshots at or after it hold no data.
   // Synchronous.
-  const double* step_snapshots(const std::vector<int>& ids, int64_t nsteps,
-                               bool graph, const std::vector<Snap>& snaps,
-                               std::vector<int64_t>& rows, int64_t& total);
+  const double* step_snapshots(
+      const std::vector<int>& ids, int64_t nsteps, bool graph,
+      const std::vector<Snap>& snaps, std::vector<int64_t>& rows,
+      int64_t& total, const std::vector<int64_t>* counts = nullptr,
+      const std::function<bool(int64_t, const double*)>& boundary = nullptr,
+      int64_t* ran = nullptr);
   // F(x) of one system into the device acceleration arrays (and optionally
   // to the host).  Drops a velocity-Verlet carry of that system.
   void accelerations(int id, double* ax, double* ay, double* az);
@@ -173,6 +187,7 @@ class Engine {
   void run_round(std::vector<WorkItem>& items, std::vector<WorkItem>& combine,
                  bool potential = false);
   void sort_items(std::vector<WorkItem>& items) const;
+  std::vector<int> by_size(const std::vector<int>& ids) const;
   void step_lists(const std::vector<int>& ids, int flip,
                   std::vector<WorkItem> (&force)[4],
                   std::vector<WorkItem> (&combine)[4]) const;
@@ -185,7 +200,7 @@ class Engine {
   template <typename Hook>
   void run_steps(const std::vector<int>& ids, int64_t nsteps, bool graph,
                  Hook&& after_step);
-  void ensure_stage(size_t doubles);
+  void ensure_stage(int which, size_t doubles);
   void check_id(int id) const;
   // One Barnes-Hut force evaluation of a system with its stage epilogue.
   void bh_round(int id, int stage, int src);
@@ -211,9 +226,14 @@ class Engine {
   size_t items_cap_ = 0;
   int32_t* d_list_ = nullptr;
   double* d_diag_ = nullptr;
-  double* d_stage_ = nullptr;   // gather staging (device)
-  double* h_stage_ = nullptr;   // gather staging (pinned host)
-  size_t stage_cap_ = 0;        // doubles
+  struct Staging {              // gather staging: device + pinned host
+    double* d = nullptr;
+    double* h = nullptr;
+    size_t cap = 0;             // doubles
+  };
+  // [0]: step_snapshots(), [1]: gather(), so a gather between the segments
+  // of a stepping call leaves that call's snapshots intact
+  Staging stage_[2];
   WorkItem* d_snap_items_ = nullptr;  // gather lists of step_snapshots()
   size_t snap_items_cap_ = 0;
   double* d_traj_ = nullptr;    // trajectory buffer of record()

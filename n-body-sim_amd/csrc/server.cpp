@@ -5,9 +5,12 @@
 // admission is first-fit in arrival order, and a request that does not fit
 // may be bypassed by later ones at most kMaxBypass times before admission
 // stops behind it (no starvation).  All resident systems advance together in
-// epochs of up to snapshot_every steps; an epoch ends early only at some
-// system's last step.  Snapshots are taken at multiples of snapshot_every
-// of each system's own steps, by gather launches between the epoch's steps,
+// epochs of snapshot_every steps; a system whose request ends inside an
+// epoch stops stepping at its last step while the others go on, and is
+// retired at the epoch's end; the epoch is cut at such a step when other
+// requests wait for admission.  Snapshots are taken at multiples of
+// snapshot_every of each system's own steps, by gather launches between the
+// epoch's steps,
 // so every system is snapshotted and retired at the same own-step counts
 // whether it runs alone or in a mix (its target reordering in MODE_F32R also
 // follows its own step count).  New requests join at the next epoch.  The
@@ -57,6 +60,7 @@ struct Active {
   int sys = -1;
   double t_admit = 0;
   bool cancel = false;
+  bool done_sent = false;  // DONE emitted inside an epoch; only removal left
 };
 
 struct Event {
@@ -313,44 +317,96 @@ class Server {
         const uint64_t id = a.req.id;
         active_.emplace(id, std::move(a));
       }
-      // retire cancelled and finished requests, then choose the epoch length:
-      // up to snapshot_every steps, ending early only where a request ends
+      // retire cancelled and finished requests; the epoch is snapshot_every
+      // steps, and a request that ends inside it stops stepping at its last
+      // step (the epoch goes on for the others)
       retire_finished();
       std::vector<int> ids;
-      int64_t k = cfg_.snapshot_every;
+      std::map<int, int64_t> left;  // system -> steps in this epoch
+      int64_t k = 0;
       {
         std::lock_guard<std::mutex> lk(mu_);
         for (auto& kv : active_) {
           const Active& a = kv.second;
+          const int64_t c = std::min<int64_t>(
+              cfg_.snapshot_every, a.req.steps - eng_->steps_done(a.sys));
           ids.push_back(a.sys);
-          k = std::min(k, a.req.steps - eng_->steps_done(a.sys));
+          left[a.sys] = c;
+          k = std::max(k, c);
         }
       }
-      if (ids.empty()) continue;
+      if (ids.empty() || k <= 0) continue;
       std::sort(ids.begin(), ids.end());
+      std::vector<int64_t> counts;
+      for (int id : ids) counts.push_back(left[id]);
       // snapshots at own-step multiples of snapshot_every, gathered on the
       // device as the epoch runs (the final state goes with DONE instead)
       std::vector<Engine::Snap> snaps;
       std::vector<const Active*> snap_req;
+      std::vector<int64_t> snap_step;  // the request's own step count
       {
         std::lock_guard<std::mutex> lk(mu_);
         for (int64_t t = 0; t < k; ++t)
           for (auto& kv : active_) {
             const Active& a = kv.second;
+            if (t >= left[a.sys]) continue;
             const int64_t T = eng_->steps_done(a.sys) + t + 1;
             if (a.req.want_snapshots && T < a.req.steps &&
                 T % cfg_.snapshot_every == 0) {
               snaps.push_back({t, a.sys});
               snap_req.push_back(&a);
+              snap_step.push_back(T);
             }
           }
       }
       std::vector<int64_t> rows;
-      int64_t total = 0;
+      int64_t total = 0, ran = k;
       const double* h = nullptr;
+      std::vector<char> snap_sent(snaps.size(), 0);
+      // queues the snapshot events of steps before t not sent yet, in step
+      // order (a request's snapshots always precede its DONE)
+      auto emit_snaps = [&](int64_t t, const double* hv) {
+        if (!hv) return;
+        std::vector<Event> evs;
+        for (size_t q = 0; q < snaps.size(); ++q) {
+          if (snap_sent[q] || snaps[q].t >= t) continue;
+          snap_sent[q] = 1;
+          const Active& a = *snap_req[q];
+          const int64_t n = a.req.p.n;
+          Event e;
+          fill(e.ev, NBODY_EVENT_SNAPSHOT, a.req, snap_step[q], a.t_admit, "");
+          e.x.assign(hv + rows[q], hv + rows[q] + n);
+          e.y.assign(hv + total + rows[q], hv + total + rows[q] + n);
+          e.z.assign(hv + 2 * total + rows[q], hv + 2 * total + rows[q] + n);
+          evs.push_back(std::move(e));
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto& e : evs) push_event(std::move(e));
+      };
       try {
-        // a graph pays off only while the resident set stays the same
-        h = eng_->step_snapshots(ids, k, ids == prev_ids, snaps, rows, total);
+        // a graph pays off only while the resident set stays the same; the
+        // epoch is cut where a request ends if others wait for admission
+        h = eng_->step_snapshots(
+            ids, k, ids == prev_ids, snaps, rows, total, &counts,
+            [&](int64_t t0, const double* hv) {
+              // requests that just took their last step report at once,
+              // after the snapshots taken so far
+              emit_snaps(t0, hv);
+              std::vector<uint64_t> fin;
+              {
+                std::lock_guard<std::mutex> lk(mu_);
+                for (auto& kv : active_) {
+                  const Active& a = kv.second;
+                  if (!a.done_sent && !a.cancel &&
+                      eng_->steps_done(a.sys) >= a.req.steps)
+                    fin.push_back(kv.first);
+                }
+              }
+              emit_done(fin);
+              std::lock_guard<std::mutex> lk(mu_);
+              return !queue_.empty();
+            },
+            &ran);
       } catch (const std::exception& ex) {
         std::lock_guard<std::mutex> lk(mu_);
         for (auto it = active_.begin(); it != active_.end();) {
@@ -369,46 +425,64 @@ class Server {
         continue;
       }
       prev_ids = ids;
-      std::vector<Event> evs;
-      for (size_t q = 0; q < snaps.size(); ++q) {
-        const Active& a = *snap_req[q];
-        const int64_t n = a.req.p.n;
-        const int64_t T = eng_->steps_done(a.sys) - k + snaps[q].t + 1;
-        Event e;
-        fill(e.ev, NBODY_EVENT_SNAPSHOT, a.req, T, a.t_admit, "");
-        e.x.assign(h + rows[q], h + rows[q] + n);
-        e.y.assign(h + total + rows[q], h + total + rows[q] + n);
-        e.z.assign(h + 2 * total + rows[q], h + 2 * total + rows[q] + n);
-        evs.push_back(std::move(e));
-      }
+      for (auto& kv : left) kv.second = std::min(kv.second, ran);
+      emit_snaps(ran, h);  // snapshots after a cut were not taken
       double inter = 0;
       std::lock_guard<std::mutex> lk(mu_);
-      epoch_steps_ += k;
+      epoch_steps_ += ran;
       for (auto& kv : active_) {
         const double n = (double)kv.second.req.p.n;
-        inter += n * (n - 1.0) * (double)(k * evals_per_step(kv.second.req.p.integ));
+        const auto c = left.find(kv.second.sys);
+        if (c == left.end()) continue;
+        inter += n * (n - 1.0) *
+                 (double)(c->second * evals_per_step(kv.second.req.p.integ));
       }
       interactions_ += inter;
-      for (auto& e : evs) push_event(std::move(e));
     }
   }
 
   // Emits the final events of cancelled and finished requests (final state
   // gathered for all of them in one pass) and frees their arena ranges.
   void retire_finished() {
-    std::vector<uint64_t> fin, canc;
+    std::vector<uint64_t> fin, canc, sent;
     {
       std::lock_guard<std::mutex> lk(mu_);
       for (auto& kv : active_) {
         const Active& a = kv.second;
-        if (a.cancel)
+        if (a.done_sent)
+          sent.push_back(kv.first);
+        else if (a.cancel)
           canc.push_back(kv.first);
         else if (eng_->steps_done(a.sys) >= a.req.steps)
           fin.push_back(kv.first);
       }
     }
-    if (fin.empty() && canc.empty()) return;
+    if (fin.empty() && canc.empty() && sent.empty()) return;
+    emit_done(fin);
     std::vector<Event> evs;
+    std::lock_guard<std::mutex> lk(mu_);
+    for (uint64_t id : canc) {
+      Active& a = active_.at(id);
+      Event e;
+      fill(e.ev, NBODY_EVENT_CANCELLED, a.req, eng_->steps_done(a.sys),
+           a.t_admit, "cancelled");
+      evs.push_back(std::move(e));
+    }
+    for (auto* lst : {&fin, &canc, &sent})
+      for (uint64_t id : *lst) {
+        try {
+          eng_->remove(active_.at(id).sys);
+        } catch (...) {
+        }
+        active_.erase(id);
+      }
+    for (auto& e : evs) push_event(std::move(e));
+  }
+
+  // DONE (or FAILED) events of the listed finished requests, their final
+  // state gathered in one pass; marks them sent.
+  void emit_done(const std::vector<uint64_t>& fin) {
+    if (fin.empty()) return;
     std::vector<int> sys;
     {
       std::lock_guard<std::mutex> lk(mu_);
@@ -418,12 +492,10 @@ class Server {
     int64_t total = 0;
     const double* h = nullptr;
     std::string err;
-    if (!sys.empty()) {
-      try {
-        h = eng_->gather(sys, 6, rows, total);
-      } catch (const std::exception& ex) {
-        err = ex.what();
-      }
+    try {
+      h = eng_->gather(sys, 6, rows, total);
+    } catch (const std::exception& ex) {
+      err = ex.what();
     }
     std::lock_guard<std::mutex> lk(mu_);
     for (size_t q = 0; q < fin.size(); ++q) {
@@ -441,24 +513,9 @@ class Server {
         fill(e.ev, NBODY_EVENT_FAILED, a.req, done, a.t_admit, err.c_str());
         ++failed_;
       }
-      evs.push_back(std::move(e));
+      a.done_sent = true;
+      push_event(std::move(e));
     }
-    for (uint64_t id : canc) {
-      Active& a = active_.at(id);
-      Event e;
-      fill(e.ev, NBODY_EVENT_CANCELLED, a.req, eng_->steps_done(a.sys),
-           a.t_admit, "cancelled");
-      evs.push_back(std::move(e));
-    }
-    for (auto* lst : {&fin, &canc})
-      for (uint64_t id : *lst) {
-        try {
-          eng_->remove(active_.at(id).sys);
-        } catch (...) {
-        }
-        active_.erase(id);
-      }
-    for (auto& e : evs) push_event(std::move(e));
   }
 
   nbody_server_config cfg_;

@@ -26,6 +26,7 @@ def drain(srv, ids, timeout_s=120.0):
             continue
         ev, data = r
         if ev["kind"] == nb.EVENT_SNAPSHOT:
+            assert ev["id"] not in done, "snapshot after the request's final event"
             snaps[ev["id"]].append((ev["step"], data))
         else:
             done[ev["id"]] = (ev, data)
@@ -55,7 +56,8 @@ def run_server(reqs, precision, snapshots=True, capacity=1 << 16):
     return ids, done, snaps, stats
 
 
-@pytest.mark.parametrize("precision", [nb.Precision.F64_STRICT, nb.Precision.F32C])
+@pytest.mark.parametrize("precision", [nb.Precision.F64_STRICT, nb.Precision.F32C,
+                                       nb.Precision.F32R])
 def test_alone_equals_mixed(gpu, precision):
     reqs = requests(20, 1)
     ids, done, snaps, stats = run_server(reqs, precision)
