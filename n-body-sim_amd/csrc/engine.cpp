@@ -61,6 +61,7 @@ Engine::~Engine() {
   }
   hipFree(d_traj_);
   hipFree(d_snap_items_);
+  hipFree(d_step_spans_);
   hipFree(d_sort_tab_);
   hipFree(d_keys_);
   hipFree(d_vals_);
@@ -551,22 +552,35 @@ void Engine::prepare(const std::vector<int>& ids) {
 // round runs, given `flip` extra steps already taken by Euler /
 // velocity-Verlet systems (RK4 systems return to the same buffer after their
 // four stages).
-void Engine::step_lists(const std::vector<int>& ids, int flip,
-                        std::vector<WorkItem> (&force)[4],
-                        std::vector<WorkItem> (&combine)[4]) const {
-  // longest systems first (the order sort_items gives), built in that order
-  const std::vector<int> order = by_size(ids);
+// The item spans of one step's lists: force and combine runs of every
+// listed system per round (RK4 systems in all four rounds, Euler and VV
+// ones in the first), longest systems first.  flip: the snapshot parity of
+// the second of two consecutive steps.
+void Engine::step_spans(const std::vector<int>& order, int flip,
+                        std::vector<ItemSpan> (&force)[4],
+                        std::vector<ItemSpan> (&combine)[4]) const {
   for (int r = 0; r < 4; ++r) {
     force[r].clear();
     combine[r].clear();
     for (int id : order) {
       const Sys& s = sys_[id];
+      int stage, src;
       if (s.p.integ == INTEG_RK4) {
-        tiles_of(id, STAGE_RK0 + r, s.cur ^ (r & 1), force[r], &combine[r]);
+        stage = STAGE_RK0 + r;
+        src = s.cur ^ (r & 1);
       } else if (r == 0) {
-        tiles_of(id, s.p.integ == INTEG_EULER ? STAGE_EULER : STAGE_VV,
-                 s.cur ^ flip, force[r], &combine[r]);
+        stage = s.p.integ == INTEG_EULER ? STAGE_EULER : STAGE_VV;
+        src = s.cur ^ flip;
+      } else {
+        continue;
       }
+      const int32_t tiles = (int32_t)((s.p.n + kWave - 1) / kWave);
+      const int parts = split_of(id);
+      force[r].push_back(ItemSpan{0, id, tiles, (int16_t)stage, (int8_t)src,
+                                  (int8_t)parts});
+      if (parts > 1)
+        combine[r].push_back(
+            ItemSpan{0, id, tiles, (int16_t)stage, (int8_t)src, (int8_t)1});
     }
   }
 }
@@ -629,36 +643,60 @@ void Engine::upload_step_lists(const std::vector<int>& ids, StepLists& L) {
     L = step_lists_;
     return;
   }
-  std::vector<WorkItem> fa[4], ca[4], fb[4], cb[4];
-  step_lists(ids, 0, fa, ca);
-  step_lists(ids, 1, fb, cb);
-  std::vector<WorkItem>& all = step_host_;
-  all.clear();
+  for (int id : ids) {
+    check_id(id);
+    const Sys& s = sys_[id];
+    if (s.off < 0 || s.off + s.span > cap_ || s.span < s.p.n)
+      throw Error(NBODY_E_STATE, "system range outside the arena");
+  }
+  const std::vector<int> order = by_size(ids);
+  std::vector<ItemSpan> fa[4], ca[4], fb[4], cb[4];
+  step_spans(order, 0, fa, ca);
+  step_spans(order, 1, fb, cb);
+  // one span array for the 16 lists; each list's items follow the previous
+  std::vector<ItemSpan> spans;
+  int64_t total = 0;
   for (int k = 0; k < 16; ++k) {
     const int which = k / 8, r = (k % 8) / 2, kind = k % 2;
-    const std::vector<WorkItem>& l =
+    const std::vector<ItemSpan>& l =
         which == 0 ? (kind ? ca[r] : fa[r]) : (kind ? cb[r] : fb[r]);
-    L.off[k] = (int)all.size();
-    L.cnt[k] = (int)l.size();
-    all.insert(all.end(), l.begin(), l.end());
+    L.off[k] = (int)total;
+    for (ItemSpan sp : l) {
+      sp.first = (int32_t)total;
+      total += (int64_t)sp.tiles * sp.parts;
+      if (total > INT32_MAX) throw Error(NBODY_E_INVALID, "work list too long");
+      spans.push_back(sp);
+    }
+    L.cnt[k] = (int)(total - L.off[k]);
   }
-  validate_items(all);
   // the previous lists may still be in use by queued launches
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  if (all.size() > step_items_cap_) {
+  if ((size_t)total > step_items_cap_) {
     hipFree(d_step_items_);
     d_step_items_ = nullptr;
-    step_items_cap_ = std::max(all.size(), 2 * step_items_cap_);
+    step_items_cap_ = std::max((size_t)total, 2 * step_items_cap_);
     hip_check(hipMalloc((void**)&d_step_items_,
                         sizeof(WorkItem) * step_items_cap_),
               "hipMalloc(step lists)");
     drop_graphs();  // graphs hold the old buffer address
   }
-  if (!all.empty()) {
-    hip_check(hipMemcpyAsync(d_step_items_, all.data(),
-                             sizeof(WorkItem) * all.size(),
+  if (spans.size() > step_spans_cap_) {
+    hipFree(d_step_spans_);
+    d_step_spans_ = nullptr;
+    step_spans_cap_ = std::max(spans.size(), 2 * step_spans_cap_);
+    hip_check(hipMalloc((void**)&d_step_spans_,
+                        sizeof(ItemSpan) * step_spans_cap_),
+              "hipMalloc(step spans)");
+  }
+  if (!spans.empty()) {
+    hip_check(hipMemcpyAsync(d_step_spans_, spans.data(),
+                             sizeof(ItemSpan) * spans.size(),
                              hipMemcpyHostToDevice, stream_),
-              "hipMemcpy(step lists)");
+              "hipMemcpy(step spans)");
+    hip_check(launch_expand(d_step_spans_, (int)spans.size(), (int)total,
+                            d_step_items_, stream_),
+              "expand launch");
+    // the spans are host memory of this call
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   }
   step_sig_ = std::move(sig);

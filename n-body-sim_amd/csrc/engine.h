@@ -188,9 +188,9 @@ class Engine {
                  bool potential = false);
   void sort_items(std::vector<WorkItem>& items) const;
   std::vector<int> by_size(const std::vector<int>& ids) const;
-  void step_lists(const std::vector<int>& ids, int flip,
-                  std::vector<WorkItem> (&force)[4],
-                  std::vector<WorkItem> (&combine)[4]) const;
+  void step_spans(const std::vector<int>& order, int flip,
+                  std::vector<ItemSpan> (&force)[4],
+                  std::vector<ItemSpan> (&combine)[4]) const;
   void prime(const std::vector<int>& ids);
   void upload_step_lists(const std::vector<int>& ids, StepLists& L);
   void launch_step(const StepLists& L, int which);
@@ -252,7 +252,8 @@ class Engine {
   // step lists: their own device buffer, reused while the signature holds
   WorkItem* d_step_items_ = nullptr;
   size_t step_items_cap_ = 0;
-  std::vector<WorkItem> step_host_;
+  ItemSpan* d_step_spans_ = nullptr;  // spans k_expand turns into the lists
+  size_t step_spans_cap_ = 0;
   std::vector<int64_t> step_sig_;
   StepLists step_lists_;
 

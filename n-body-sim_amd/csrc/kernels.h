@@ -43,6 +43,9 @@ hipError_t launch_sort_scatter(const Arena* A, const SortSys* tab, int nsys,
 hipError_t sort_pairs(void* tmp, size_t& bytes, const uint64_t* kin,
                       uint64_t* kout, const int32_t* vin, int32_t* vout, int n,
                       int end_bit, hipStream_t s);
+// Expands nspans item spans (covering items [0, total) in order) into out.
+hipError_t launch_expand(const ItemSpan* spans, int nspans, int total,
+                         WorkItem* out, hipStream_t s);
 hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s);
 hipError_t launch_probe(int* out, hipStream_t s);
 

@@ -732,6 +732,33 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Arena* Ap,
 }
 
 // Writes zeros; used instead of hipMemsetAsync so it can sit in a graph.
+// Expands item spans into work items: thread i finds the span holding item
+// i (the spans cover [0, total) in order) by binary search.
+__global__ __launch_bounds__(256) void k_expand(const ItemSpan* spans,
+                                                int nspans, int total,
+                                                WorkItem* out) {
+  const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (i >= total) return;
+  int lo = 0, hi = nspans - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (spans[mid].first <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const ItemSpan sp = spans[lo];
+  const int k = i - sp.first;
+  WorkItem w;
+  w.sys = sp.sys;
+  w.tile = k / sp.parts;
+  w.stage = sp.stage;
+  w.src = sp.src;
+  w.part = (int8_t)(k % sp.parts);
+  w.pad_ = 0;
+  out[i] = w;
+}
+
 __global__ void k_fill_zero(double* p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
@@ -954,6 +981,14 @@ hipError_t sort_pairs(void* tmp, size_t& bytes, const uint64_t* kin,
                       int end_bit, hipStream_t s) {
   return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, kin, kout, vin, vout,
                                             n, 0, end_bit, s);
+}
+
+hipError_t launch_expand(const ItemSpan* spans, int nspans, int total,
+                         WorkItem* out, hipStream_t s) {
+  if (total <= 0 || nspans <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_expand, dim3((total + 255) / 256), dim3(256), 0, s,
+                     spans, nspans, total, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s) {

@@ -85,6 +85,19 @@ struct WorkItem {      // one wave: 64 consecutive targets of one system
 };
 static_assert(sizeof(WorkItem) == 16, "WorkItem layout");
 
+// A run of work items of one system, expanded on the device (k_expand):
+// item first + k is (sys, tile k / parts, stage, src, part k % parts) for
+// k < tiles * parts, in the order the host would list them.
+struct ItemSpan {
+  int32_t first;       // index of the run's first item
+  int32_t sys;
+  int32_t tiles;
+  int16_t stage;
+  int8_t src;
+  int8_t parts;        // source-range parts (1 for combine items)
+};
+static_assert(sizeof(ItemSpan) == 16, "ItemSpan layout");
+
 struct Arena {         // device pointers, passed by value to every kernel
   double *m, *gm64;
   float *gm32;

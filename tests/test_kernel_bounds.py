@@ -159,3 +159,35 @@ def test_sort_keys_leave_room_for_the_rank():
     top = (1 << 16) - 1
     assert hilbert48(top, top, top) < (1 << 48)
     assert hilbert48(0, 0, 0) == 0
+
+
+# ---- work lists expanded on the device (kernels.hip k_expand) ---------------
+def expand(spans, total):
+    """Model of k_expand: item i -> the last span with first <= i (binary
+    search), then (tile, part) = divmod(i - first, parts)."""
+    out = []
+    for i in range(total):
+        lo, hi = 0, len(spans) - 1
+        while lo < hi:
+            mid = (lo + hi + 1) >> 1
+            if spans[mid][0] <= i:
+                lo = mid
+            else:
+                hi = mid - 1
+        first, sys, tiles, parts = spans[lo]
+        k = i - first
+        assert 0 <= k < tiles * parts          # inside its span
+        out.append((sys, k // parts, k % parts))
+    return out
+
+
+def test_expand_matches_per_system_item_order():
+    """Spans laid out back to back expand to the host's per-system loop
+    (tiles outer, parts inner), including one-tile and one-part systems."""
+    systems = [(3, 5, 4), (0, 1, 1), (7, 2, 32), (1, 9, 1), (4, 1, 8)]
+    spans, total, ref = [], 0, []
+    for sys, tiles, parts in systems:
+        spans.append((total, sys, tiles, parts))
+        total += tiles * parts
+        ref += [(sys, t, q) for t in range(tiles) for q in range(parts)]
+    assert expand(spans, total) == ref
