@@ -46,14 +46,16 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
 
 // x^(-3/2) in binary64 with the refinement folded into the cube: with the
 // v_rsq_f64 seed y and e = 1 - x*y^2 (|e| about 2^-22),
-//   (y (1 - e)^(-1/2))^3 = y^3 (1 - e)^(-3/2) = y^3 (1 + 1.5 e + O(e^2))
-//                        = y^3 (2.5 - 1.5 x y^2) + O(e^2),
+//   (y (1 - e)^(-1/2))^3 = y^3 (1 - e)^(-3/2) = y^3 (1 + 1.5 e + O(e^2)),
 // so the cube takes the correction as one factor instead of refining y
-// first (the dropped e^2 term is below 2^-43 relative).
+// first (the dropped e^2 term is below 2^-43 relative).  Six f64 operations;
+// e comes from one fused multiply-add against the inline constant 1.0, and
+// the factor needs only one non-inline constant (one scalar operand).
 __device__ __forceinline__ double inv_cube_f64(double x) {
   const double y = __builtin_amdgcn_rsq(x);
   const double u = y * y;
-  const double c = fma(-1.5, x * u, 2.5);
+  const double e = fma(-x, u, 1.0);
+  const double c = fma(e, 1.5, 1.0);
   return (y * u) * c;
 }
 
