@@ -172,9 +172,9 @@ int nbody_ctx_create(int device, nbody_ctx** out) {
     if (e == hipSuccess)
       e = hipMemcpyAsync(&h, d, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(d);
+    (void)hipFree(d);
     if (e != hipSuccess || h != 0x6e626479) {
-      hipStreamDestroy(c->stream);
+      (void)hipStreamDestroy(c->stream);
       throw Error(NBODY_E_DEVICE,
                   std::string("GPU initialisation check failed: ") +
                       hipGetErrorString(e) +

@@ -31,11 +31,11 @@ static int split_for(int64_t n);
 
 Engine::Engine(int device, hipStream_t stream, int mode, int64_t capacity,
                int max_systems)
-    : device_(device),
-      stream_(stream),
+    : stream_(stream),
       mode_(mode),
       cap_(round_up(std::max<int64_t>(capacity, 1), kAlign)),
       max_sys_(std::max(max_systems, 1)) {
+  (void)device;  // the stream belongs to it; every call runs on the stream
   if (mode < MODE_F64_STRICT || mode > MODE_F32R)
     throw Error(NBODY_E_INVALID, "unknown precision");
   alloc_arena();
@@ -52,25 +52,25 @@ Engine::Engine(int device, hipStream_t stream, int mode, int64_t capacity,
 
 Engine::~Engine() {
   drop_graphs();
-  hipStreamSynchronize(stream_);
-  hipFree(d_items_);
-  hipFree(d_step_items_);
+  (void)hipStreamSynchronize(stream_);
+  (void)hipFree(d_items_);
+  (void)hipFree(d_step_items_);
   for (Staging& st : stage_) {
-    hipFree(st.d);
-    hipHostFree(st.h);
+    (void)hipFree(st.d);
+    (void)hipHostFree(st.h);
   }
-  hipFree(d_traj_);
-  hipFree(d_snap_items_);
-  hipFree(d_step_spans_);
-  hipFree(d_sort_tab_);
-  hipFree(d_keys_);
-  hipFree(d_vals_);
-  hipFree(d_sort_tmp_);
-  hipFree(d_diag_);
-  hipFree(d_list_);
-  hipFree(d_sys_);
-  hipFree(d_arena_);
-  hipFree(arena_mem_);
+  (void)hipFree(d_traj_);
+  (void)hipFree(d_snap_items_);
+  (void)hipFree(d_step_spans_);
+  (void)hipFree(d_sort_tab_);
+  (void)hipFree(d_keys_);
+  (void)hipFree(d_vals_);
+  (void)hipFree(d_sort_tmp_);
+  (void)hipFree(d_diag_);
+  (void)hipFree(d_list_);
+  (void)hipFree(d_sys_);
+  (void)hipFree(d_arena_);
+  (void)hipFree(arena_mem_);
 }
 
 // Carves every arena array out of one block starting at `base`; returns the
@@ -170,7 +170,7 @@ void Engine::validate_items(const std::vector<WorkItem>& items) const {
 void Engine::drop_graphs() {
   for (auto& g : gexec_)
     if (g) {
-      hipGraphExecDestroy(g);
+      (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
   gkey_ = -1;
@@ -450,7 +450,7 @@ void Engine::upload_table() {
 void Engine::ensure_items(size_t n) {
   if (n <= items_cap_) return;
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  if (d_items_) hipFree(d_items_);
+  if (d_items_) (void)hipFree(d_items_);
   d_items_ = nullptr;
   items_cap_ = std::max<size_t>(n, 2 * items_cap_);
   hip_check(hipMalloc((void**)&d_items_, sizeof(WorkItem) * items_cap_),
@@ -672,7 +672,7 @@ void Engine::upload_step_lists(const std::vector<int>& ids, StepLists& L) {
   // the previous lists may still be in use by queued launches
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   if ((size_t)total > step_items_cap_) {
-    hipFree(d_step_items_);
+    (void)hipFree(d_step_items_);
     d_step_items_ = nullptr;
     step_items_cap_ = std::max((size_t)total, 2 * step_items_cap_);
     hip_check(hipMalloc((void**)&d_step_items_,
@@ -681,7 +681,7 @@ void Engine::upload_step_lists(const std::vector<int>& ids, StepLists& L) {
     drop_graphs();  // graphs hold the old buffer address
   }
   if (spans.size() > step_spans_cap_) {
-    hipFree(d_step_spans_);
+    (void)hipFree(d_step_spans_);
     d_step_spans_ = nullptr;
     step_spans_cap_ = std::max(spans.size(), 2 * step_spans_cap_);
     hip_check(hipMalloc((void**)&d_step_spans_,
@@ -768,13 +768,13 @@ void Engine::run_steps(const std::vector<int>& ids, int64_t nsteps,
         try {
           launch_step(L, which);
         } catch (...) {
-          hipStreamEndCapture(stream_, &g);
-          if (g) hipGraphDestroy(g);
+          (void)hipStreamEndCapture(stream_, &g);
+          if (g) (void)hipGraphDestroy(g);
           throw;
         }
         hip_check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
         hipError_t e = hipGraphInstantiate(&gexec_[which], g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         hip_check(e, "hipGraphInstantiate");
       }
       gkey_ = key;
@@ -796,8 +796,8 @@ void Engine::ensure_stage(int which, size_t need) {
   Staging& st = stage_[which];
   if (need <= st.cap) return;
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  hipFree(st.d);
-  hipHostFree(st.h);
+  (void)hipFree(st.d);
+  (void)hipHostFree(st.h);
   st.d = nullptr;
   st.h = nullptr;
   st.cap = std::max(need, 2 * st.cap);
@@ -877,7 +877,7 @@ const double* Engine::step_snapshots(const std::vector<int>& ids,
     ensure_stage(0, (size_t)total * 3);
     if (items.size() > snap_items_cap_) {
       hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-      hipFree(d_snap_items_);
+      (void)hipFree(d_snap_items_);
       d_snap_items_ = nullptr;
       snap_items_cap_ = std::max(items.size(), 2 * snap_items_cap_);
       hip_check(hipMalloc((void**)&d_snap_items_,
@@ -1094,7 +1094,7 @@ void Engine::record(int id, int64_t nsteps, double* x, double* y, double* z,
   C = std::max<int64_t>(1, std::min(C, nsteps));
   if ((size_t)(C * per_step) > traj_cap_) {
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    hipFree(d_traj_);
+    (void)hipFree(d_traj_);
     d_traj_ = nullptr;
     traj_cap_ = (size_t)(C * per_step);
     hip_check(hipMalloc((void**)&d_traj_, sizeof(double) * traj_cap_),
@@ -1202,7 +1202,7 @@ std::vector<Engine::SortPass> Engine::plan_sorts(const std::vector<int>& ids,
 void Engine::upload_sort_tab(const std::vector<SortSys>& tab) {
   if (tab.size() > sort_tab_cap_) {
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    hipFree(d_sort_tab_);
+    (void)hipFree(d_sort_tab_);
     d_sort_tab_ = nullptr;
     sort_tab_cap_ = std::max(tab.size(), 2 * sort_tab_cap_);
     hip_check(hipMalloc((void**)&d_sort_tab_, sizeof(SortSys) * sort_tab_cap_),

@@ -207,7 +207,6 @@ class Engine {
   // One step of every listed system with Barnes-Hut forces (parity t & 1).
   void bh_step(const std::vector<int>& ids, int64_t t);
 
-  int device_;
   hipStream_t stream_;
   int mode_;
   int gravity_ = GRAV_DIRECT;

@@ -109,7 +109,7 @@ class Server {
       eng_ = std::make_unique<Engine>(cfg.device, stream_, mode_,
                                       cfg.capacity_bodies, cfg.max_systems);
     } catch (...) {
-      hipStreamDestroy(stream_);
+      (void)hipStreamDestroy(stream_);
       throw;
     }
     worker_ = std::thread([this] { run(); });
@@ -123,9 +123,9 @@ class Server {
     cv_work_.notify_all();
     cv_events_.notify_all();
     if (worker_.joinable()) worker_.join();
-    hipSetDevice(cfg_.device);
+    (void)hipSetDevice(cfg_.device);
     eng_.reset();
-    hipStreamDestroy(stream_);
+    (void)hipStreamDestroy(stream_);
   }
 
   uint64_t submit(const nbody_request& r) {
@@ -277,7 +277,7 @@ class Server {
   }
 
   void run() {
-    hipSetDevice(cfg_.device);
+    (void)hipSetDevice(cfg_.device);
     std::vector<int> prev_ids;
     for (;;) {
       std::vector<Request> admit;
