@@ -4,7 +4,10 @@ For every bucket b (systems of n with floor(log2 n) == b, sampled at
 n = 2^b and n = 1.5 * 2^b), each candidate part count k in {1, 2, 4, 8, 16, 32} (with
 at least 64 sources per part) is timed on a Plummer sphere stepped with
 velocity Verlet in both f32-compensated forms (f32r, f32c) and the f64
-arithmetic.  The k with the smallest summed time wins.  The table is written as
+arithmetic.  The clocks are warmed first, and every candidate is timed twice
+(candidates in increasing, then decreasing order; the faster time counts),
+so a clock ramp cannot favour the candidates timed last.  The k with the
+smallest summed time wins.  The table is written as
 `split <bucket> <parts>` lines (the format the library loads at start-up
 from tuning/gfx950.txt next to libnbody_amd.so).
 
@@ -27,7 +30,7 @@ CANDIDATES = (1, 2, 4, 8, 16, 32)
 TUNED = (nb.Precision.F32R, nb.Precision.F32C, nb.Precision.F64)
 
 
-def time_case(gpu, bodies, prec, budget_ms=40.0):
+def time_case(gpu, bodies, prec, budget_ms=120.0):
     """ms per velocity-Verlet step (one force evaluation per step)."""
     lib = nb.load_library()
     db = nb.DeviceBodies(gpu, bodies)
@@ -52,24 +55,27 @@ def tune(max_bucket=20, min_bucket=6, log=print):
     gpu = nb.GpuContext(0)
     table = nb.tuning_split()
     report = []
+    warm = nb.plummer(65536, 1)
+    t_end = time.time() + 1.0
+    while time.time() < t_end:           # bring the clocks up
+        time_case(gpu, warm, nb.Precision.F64, budget_ms=200.0)
     for b in range(min_bucket, max_bucket + 1):
         sizes = [1 << b] + ([3 << (b - 1)] if b < max_bucket else [])
         systems = [nb.plummer(n, 7 + n) for n in sizes]
-        best, best_t = table[b], None
-        row = []
-        for k in CANDIDATES:
-            if min(sizes) // k < 64:
-                continue
-            trial = list(table)
-            trial[b] = k
-            nb.set_tuning_split(trial)
-            t = 0.0
-            for bodies in systems:
-                for prec in TUNED:
-                    t += time_case(gpu, bodies, prec)
-            row.append((k, t))
-            if best_t is None or t < best_t:
-                best, best_t = k, t
+        cands = [k for k in CANDIDATES if min(sizes) // k >= 64]
+        times = {}
+        for order in (cands, cands[::-1]):
+            for k in order:
+                trial = list(table)
+                trial[b] = k
+                nb.set_tuning_split(trial)
+                t = 0.0
+                for bodies in systems:
+                    for prec in TUNED:
+                        t += time_case(gpu, bodies, prec)
+                times[k] = min(times.get(k, t), t)
+        row = [(k, times[k]) for k in cands]
+        best = min(cands, key=lambda k: times[k])
         table[b] = best
         nb.set_tuning_split(table)
         log(f"bucket {b:2d} (n={sizes}): " +
