@@ -222,7 +222,10 @@ int nbody_batch_destroy(nbody_batch *bt);
 /* ---- served form: continuous batching of simulation requests.
  * Requests are admitted by device-memory footprint as slots free up, packed
  * into shared steps with everything else resident, and stream position
- * snapshots back every snapshot_every steps. */
+ * snapshots back every snapshot_every steps.  A request's events arrive in
+ * step order, its snapshots all before its DONE; DONE is queued as soon as
+ * the request's last step has run.  Results are the same bits whatever
+ * else is resident. */
 typedef struct {
   int device;
   int precision;              /* NBODY_PRECISION_* for every request */
