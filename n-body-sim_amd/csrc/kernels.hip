@@ -660,17 +660,24 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const Arena* Ap,
 // summation order (thread t takes bodies t, t+256, ...; then a fixed LDS
 // tree), so results are reproducible bit for bit.  Needs phi from a
 // potential pass.  out[s*9 + DIAG_*].
-__global__ __launch_bounds__(kBlock) void k_diag(const Arena* Ap, const SysDesc* sysd,
-                                                 const int32_t* list,
-                                                 double* out) {
-  __shared__ double red[DIAG_COUNT][kBlock];
+// One workgroup of kDiagBlock threads per listed system; thread t sums the
+// bodies t, t + kDiagBlock, ..., then each wave reduces by a fixed
+// butterfly and thread q adds the waves' values of quantity q in wave order:
+// a fixed order, so the result is reproducible.
+constexpr int kDiagBlock = 1024;
+__global__ __launch_bounds__(kDiagBlock) void k_diag(const Arena* Ap,
+                                                     const SysDesc* sysd,
+                                                     const int32_t* list,
+                                                     double* out) {
+  constexpr int kWaves = kDiagBlock / kWave;
+  __shared__ double red[DIAG_COUNT][kWaves];
   const int s = list[blockIdx.x];
   const CArena* A = (const CArena*)Ap;
   const CSys* sd = (const CSys*)(sysd + s);
   double acc[DIAG_COUNT];
 #pragma unroll
   for (int q = 0; q < DIAG_COUNT; ++q) acc[q] = 0.0;
-  for (int64_t k = threadIdx.x; k < sd->n; k += kBlock) {
+  for (int64_t k = threadIdx.x; k < sd->n; k += kDiagBlock) {
     const int64_t i = sd->off + k;
     const double m = A->m[i];
     const double vx = A->vx[i], vy = A->vy[i], vz = A->vz[i];
@@ -685,18 +692,17 @@ __global__ __launch_bounds__(kBlock) void k_diag(const Arena* Ap, const SysDesc*
     acc[DIAG_M] += m;
   }
 #pragma unroll
-  for (int q = 0; q < DIAG_COUNT; ++q) red[q][threadIdx.x] = acc[q];
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
+  for (int q = 0; q < DIAG_COUNT; ++q)
+    for (int d = kWave / 2; d > 0; d >>= 1) acc[q] += __shfl_xor(acc[q], d, kWave);
+  const int wave = (int)(threadIdx.x / kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) {
 #pragma unroll
-      for (int q = 0; q < DIAG_COUNT; ++q)
-        red[q][threadIdx.x] += red[q][threadIdx.x + w];
-    }
-    __syncthreads();
+    for (int q = 0; q < DIAG_COUNT; ++q) red[q][wave] = acc[q];
   }
+  __syncthreads();
   if (threadIdx.x < DIAG_COUNT) {
     double v = red[threadIdx.x][0];
+    for (int w = 1; w < kWaves; ++w) v += red[threadIdx.x][w];
     if (threadIdx.x == DIAG_U) v = -0.5 * v;  // each pair counted twice
     out[(int64_t)blockIdx.x * DIAG_COUNT + threadIdx.x] = v;
   }
@@ -944,7 +950,7 @@ hipError_t launch_prepare(const Arena* A, const WorkItem* items, int nitems,
 hipError_t launch_diag(const Arena* A, const SysDesc* sys, const int32_t* list,
                        int nlist, double* out, hipStream_t s) {
   if (nlist <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_diag, dim3(nlist), dim3(kBlock), 0, s, A, sys, list,
+  hipLaunchKernelGGL(k_diag, dim3(nlist), dim3(kDiagBlock), 0, s, A, sys, list,
                      out);
   return hipGetLastError();
 }
