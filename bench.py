@@ -63,10 +63,24 @@ def cpu_baseline(b, g, eps, seconds):
             break
     dt = time.perf_counter() - t0
     rate = n * (n - 1) * reps / dt
+    # ms per force evaluation at other sizes: median of 5 after one warm-up
+    # (seeded Plummer spheres); N = 1M extrapolated from the N above
+    per_eval = {}
+    for m in (1024, 8192):
+        p = nb.plummer(m, 2)
+        ts = []
+        for _ in range(6):
+            t1 = time.perf_counter()
+            oracle.newton_parallel(p.masses, p.pos_x, p.pos_y, p.pos_z, g, eps, threads)
+            ts.append(time.perf_counter() - t1)
+        per_eval[str(m)] = 1e3 * float(np.median(ts[1:]))
+    per_eval[str(n)] = 1e3 * dt / reps
+    per_eval["1048576 (extrapolated)"] = 1e3 * (1 << 20) * ((1 << 20) - 1) / rate
     return {"value": rate, "unit": "pair-interactions/s", "cores": threads,
             "kind": "port",
             "sample": f"{reps} full force evaluation(s) of the same N={n} system "
-                      f"(oracle newton-parallel, {threads} threads, {dt:.1f} s)"}, acc
+                      f"(oracle newton-parallel, {threads} threads, {dt:.1f} s)",
+            "ms_per_eval": per_eval}, acc
 
 
 def time_single(gpu, b, prec, g, eps, dt, steps, warmup):
@@ -156,12 +170,16 @@ def run_single(args, gpu):
         cpu, acc_ref = cpu_baseline(b, g, eps, args.cpu_seconds)
     acc_rep = {}
     drift = lambda e0, e1: abs(e1[2] - e0[2]) / abs(e0[2])  # noqa: E731
+    # momentum drift |P(t) - P(0)| / sum_i m_i |v_i(0)|
+    pscale = float(np.sum(b.masses * np.sqrt(b.vel_x ** 2 + b.vel_y ** 2 + b.vel_z ** 2)))
+    pdrift = lambda e0, e1: float(np.linalg.norm(e1[3:6] - e0[3:6])) / pscale  # noqa: E731
     fin64 = res["f64"][4]
     for k in forms:
         _, acc0, e0, e1, fin = res[k]
         if acc_ref is not None:
             acc_rep[f"{k}_max_rel"], acc_rep[f"{k}_median_rel"] = rel_err(acc0, acc_ref)
         acc_rep[f"{k}_energy_drift"] = drift(e0, e1)
+        acc_rep[f"{k}_momentum_drift"] = pdrift(e0, e1)
         if k != "f64":
             # trajectories after warmup + steps against f64, in units of the
             # Plummer scale radius (1)
