@@ -1,17 +1,24 @@
 // gfx950 kernels for the direct-sum gravity step.
 //
-// One wavefront owns one work item: 64 consecutive target bodies of one
-// system, one target per lane.  The wave walks that system's source bodies in
-// index order.  Source addresses are wave-uniform, so sources are read through
-// address-space-4 pointers and land in SGPRs via s_load (the scalar cache
-// broadcasts them; no LDS, no barriers).  Each lane keeps its own running
-// sums, so the j order per target is the reference's sequential order
-// (src/gravity/cpu/newton.rs:104-124): j = 0..n-1 with j == i skipped.
+// One wavefront owns one work item: 64 consecutive target slots of one
+// system (one target per lane) and one part of its source range.
+//   * F64_STRICT walks all sources in index order, one part, so each target
+//     sums j = 0..n-1 with j == i skipped, the reference's sequential order
+//     (src/gravity/cpu/newton.rs:104-124).
+//   * F64 and F32C read sources through address-space-4 pointers: the
+//     addresses are wave-uniform, so they land in SGPRs via s_load and the
+//     scalar cache broadcasts them (no LDS, no barriers).
+//   * F32R walks targets and sources in Hilbert order; each 64-source chunk
+//     is differenced against the wave's origin once, one source per lane,
+//     and staged in the wave's own LDS slice.
+// With the source range split into parts, every part writes partial sums
+// and k_combine adds them in part order, so results do not depend on
+// scheduling.
 //
-// The integrator stage for the system is fused into the epilogue of the same
-// launch: the wave updates its own bodies' master state and writes their next
-// source snapshot into the other half of a double buffer, so no target of the
-// running launch ever reads a value this launch writes.
+// The integrator stage is fused into the epilogue (of the force launch, or
+// of k_combine when split): it updates the target's master state and writes
+// its next source snapshot into the other half of a double buffer, so no
+// target of a running launch ever reads a value that launch writes.
 #include <hip/hip_runtime.h>
 
 #include <hipcub/hipcub.hpp>
@@ -332,8 +339,7 @@ struct TargetR {
 // into its own LDS slice and a wave's LDS operations are performed in the
 // order issued, so no workgroup barrier is needed; this only stops the
 // compiler from moving memory accesses across the point.  (A workgroup
-// barrier would also wait for every outstanding global load, the prefetched
-// next chunk included.)
+// barrier would also make the wave wait for every outstanding global load.)
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
