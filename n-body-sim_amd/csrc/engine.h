@@ -153,14 +153,15 @@ class Engine {
     double com[3] = {0, 0, 0};  // centre of mass at upload (sort box centre,
     double vcom[3] = {0, 0, 0};  // advanced with the centre-of-mass velocity)
     double ext = 0.0;           // largest coordinate distance from com
-    int64_t sorted_at = -1;     // step count of the last target ordering
+    int64_t sorted_at = -1;     // step count of the last body ordering
   };
-  // MODE_F32R target ordering.  A system's targets are put in Hilbert order
-  // before its first F32C evaluation and before every step whose count is a
-  // multiple of kResortEvery -- a schedule of the system's own step count,
-  // so results do not depend on how steps are split across calls or on the
-  // other systems stepped with it.  plan_sorts() uploads the tables of every
-  // pass one call needs (pass t runs before that call's step t).
+  // MODE_F32R body ordering.  A system's bodies are put in Hilbert order
+  // (perm, and the slot-ordered snapshot copy) before its first F32R
+  // evaluation and before every step whose count is a multiple of
+  // kResortEvery -- a schedule of the system's own step count, so results do
+  // not depend on how steps are split across calls or on the other systems
+  // stepped with it.  plan_sorts() uploads the tables of every pass one call
+  // needs (pass t runs before that call's step t).
   struct SortPass {
     int64_t t;
     int nsys, max_tiles, total, tab;
@@ -237,7 +238,7 @@ class Engine {
   size_t snap_items_cap_ = 0;
   double* d_traj_ = nullptr;    // trajectory buffer of record()
   size_t traj_cap_ = 0;         // doubles
-  // target ordering (MODE_F32C): keys/values in and out, sort scratch
+  // body ordering (MODE_F32R): keys/values in and out, sort scratch
   SortSys* d_sort_tab_ = nullptr;
   size_t sort_tab_cap_ = 0;
   uint64_t* d_keys_ = nullptr;  // 2 * cap: in, out
