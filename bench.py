@@ -141,6 +141,13 @@ DTYPE = {
 }
 
 
+def default_f32(workload):
+    """The faster f32 form for a workload: many small systems (a few chunks
+    each, most of them near their targets) favour the per-pair f64 difference
+    of f32c; one large system favours f32r."""
+    return "f32c" if workload in ("batched", "serve") else "f32r"
+
+
 def f32_forms(args):
     """The headline f32 form first, then the other one."""
     other = "f32c" if args.f32 == "f32r" else "f32r"
@@ -549,9 +556,7 @@ def main():
               "positions per force evaluation -- not built here)", file=sys.stderr)
         sys.exit(2)
     if args.f32 is None:
-        # many small systems (a few chunks each, most of them near their
-        # targets) favour the per-pair f64 difference of f32c
-        args.f32 = "f32c" if args.workload in ("batched", "serve") else "f32r"
+        args.f32 = default_f32(args.workload)
     if args.steps is None:
         args.steps = {"single": 200, "batched": 50, "serve": 0, "large": 2,
                       "bh": 5, "copy": 1}[args.workload]
