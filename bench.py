@@ -83,6 +83,21 @@ def cpu_baseline(b, g, eps, seconds):
             "ms_per_eval": per_eval}, acc
 
 
+def pair_scale(b, g, eps, idx):
+    """S_i = sum_j |a_ij| for the listed targets (numpy f64): the size of the
+    terms summed into a_i, against which an error shows the conditioning."""
+    pos = np.stack([b.pos_x, b.pos_y, b.pos_z], 1)
+    out = np.empty(len(idx))
+    for k in range(0, len(idx), 32):
+        sel = idx[k:k + 32]
+        d = pos[None, :, :] - pos[sel][:, None, :]
+        r2 = (d * d).sum(2)
+        w = g * b.masses[None, :] * np.sqrt(r2) / (r2 + eps * eps) ** 1.5
+        w[np.arange(len(sel)), sel] = 0.0
+        out[k:k + 32] = w.sum(1)
+    return out
+
+
 def time_single(gpu, b, prec, g, eps, dt, steps, warmup):
     grav = nb.GpuNewtonParallelGravity(g, eps * eps, prec)
     integ = nb.GpuVelocityVerletIntegrator(dt)
@@ -181,10 +196,18 @@ def run_single(args, gpu):
     pscale = float(np.sum(b.masses * np.sqrt(b.vel_x ** 2 + b.vel_y ** 2 + b.vel_z ** 2)))
     pdrift = lambda e0, e1: float(np.linalg.norm(e1[3:6] - e0[3:6])) / pscale  # noqa: E731
     fin64 = res["f64"][4]
+    if acc_ref is not None:
+        # normalised error |da_i| / sum_j |a_ij| on a seeded target sample
+        sample = np.sort(np.random.default_rng(5).choice(n, 2048, replace=False))
+        scale = pair_scale(b, g, eps, sample)
+        ref_s = np.stack(acc_ref, 1)[sample]
     for k in forms:
         _, acc0, e0, e1, fin = res[k]
         if acc_ref is not None:
             acc_rep[f"{k}_max_rel"], acc_rep[f"{k}_median_rel"] = rel_err(acc0, acc_ref)
+            dn = np.linalg.norm(np.stack(acc0, 1)[sample] - ref_s, axis=1) / scale
+            acc_rep[f"{k}_max_norm_sampled"] = float(dn.max())
+            acc_rep[f"{k}_median_norm_sampled"] = float(np.median(dn))
         acc_rep[f"{k}_energy_drift"] = drift(e0, e1)
         acc_rep[f"{k}_momentum_drift"] = pdrift(e0, e1)
         if k != "f64":
