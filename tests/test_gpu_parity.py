@@ -326,3 +326,34 @@ def test_batch_matches_oracle(gpu):
         o, n = bt.offsets[s], bt.n[s]
         for q in range(6):
             assert np.array_equal(out[q][o:o + n], ref[q]), (s, q)
+
+
+@pytest.mark.parametrize("prec", [nb.Precision.F32R, nb.Precision.F32C, nb.Precision.F64])
+def test_g_change_rebuilds_every_copy_of_gm(gpu, prec):
+    """A new G on the same device bodies reaches every copy of g*m (for
+    F32R also the slot-ordered one): G = 2 doubles each term exactly, so the
+    accelerations double bit for bit."""
+    b = nb.plummer(3000, 17)
+    db = nb.DeviceBodies(gpu, b)
+    a1 = nb.GpuNewtonParallelGravity(1.0, 1e-4, prec).calculate_accelerations(gpu, db)
+    a2 = nb.GpuNewtonParallelGravity(2.0, 1e-4, prec).calculate_accelerations(gpu, db)
+    db.close()
+    for q in range(3):
+        assert np.array_equal(a2[q], 2.0 * a1[q]), q
+
+
+def test_precision_switch_on_the_same_bodies(gpu):
+    """F32R, then F64, then F32R again on one set of device bodies: the
+    ordering is rebuilt on the switch back and gives the first result's
+    bits; the F64 result in between meets its own contract."""
+    b = nb.plummer(5000, 18)
+    ref = oracle.newton_parallel(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, 0.01)
+    db = nb.DeviceBodies(gpu, b)
+    r1 = nb.GpuNewtonParallelGravity(1.0, 1e-4, nb.Precision.F32R).calculate_accelerations(gpu, db)
+    f64 = nb.GpuNewtonParallelGravity(1.0, 1e-4, nb.Precision.F64).calculate_accelerations(gpu, db)
+    r2 = nb.GpuNewtonParallelGravity(1.0, 1e-4, nb.Precision.F32R).calculate_accelerations(gpu, db)
+    db.close()
+    for q in range(3):
+        assert np.array_equal(r1[q], r2[q]), q
+    assert rel_err(f64, ref).max() <= 1e-12
+    assert rel_err(r1, ref).max() <= 1e-5
