@@ -57,6 +57,8 @@ extern "C" {
 #define NBODY_PRECISION_F64 1         /* f64, rsq + refinement, FMA; same j order */
 #define NBODY_PRECISION_F32C 2        /* f64 positions, each difference rounded once to f32, f32 pair terms, compensated sums; f64 state */
 #define NBODY_PRECISION_F32R 3        /* bodies in Hilbert order, coordinates relative to each wave's middle target split into an f32 value and an f32 residual; f32 pair terms, the residuals added for source chunks near the targets; compensated sums; <= 1e-5 relative; faster than F32C for large systems */
+/* The f32 forms need pair distances below about 1e18 (r^2 in f32 range);
+ * wider systems need NBODY_PRECISION_F64. */
 
 typedef struct nbody_ctx nbody_ctx;
 typedef struct nbody_bodies nbody_bodies;
