@@ -45,6 +45,10 @@ Engine::Engine(int device, hipStream_t stream, int mode, int64_t capacity,
             "hipMalloc(system list)");
   hip_check(hipMalloc((void**)&d_diag_, sizeof(double) * DIAG_COUNT * max_sys_),
             "hipMalloc(diagnostics)");
+  hip_check(hipEventCreateWithFlags(&ev_gather_, hipEventDisableTiming),
+            "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&ev_boundary_, hipEventDisableTiming),
+            "hipEventCreate");
   h_sys_.assign(max_sys_, SysDesc{});
   sys_.assign(max_sys_, Sys{});
   free_[0] = cap_;
@@ -71,6 +75,8 @@ Engine::~Engine() {
   (void)hipFree(d_sys_);
   (void)hipFree(d_arena_);
   (void)hipFree(arena_mem_);
+  if (ev_gather_) (void)hipEventDestroy(ev_gather_);
+  if (ev_boundary_) (void)hipEventDestroy(ev_boundary_);
 }
 
 // Carves every arena array out of one block starting at `base`; returns the
@@ -814,7 +820,8 @@ const double* Engine::step_snapshots(const std::vector<int>& ids,
                                      std::vector<int64_t>& rows,
                                      int64_t& total,
                                      const std::vector<int64_t>* counts,
-                                     const std::function<bool(int64_t, const double*)>& boundary,
+                                     const std::function<bool(int64_t)>& boundary,
+                                     const std::function<void(int64_t, const double*)>& deliver,
                                      int64_t* ran) {
   rows.clear();
   total = 0;
@@ -890,9 +897,10 @@ const double* Engine::step_snapshots(const std::vector<int>& ids,
               "hipMemcpy(snapshot lists)");
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   }
-  // rows [0, on_host) of every coordinate are on the host (step order)
+  // rows [0, on_host) of every coordinate are copied to the host, or queued
+  // to be (step order)
   int64_t on_host = 0;
-  auto to_host_before = [&](int64_t t) {  // snapshots taken before step t
+  auto copy_before = [&](int64_t t) {  // snapshots taken before step t
     size_t k = 0;
     while (k < order.size() && order[k].t < t) ++k;
     const int64_t end = k < order.size() ? srow[k] : total;
@@ -903,7 +911,6 @@ const double* Engine::step_snapshots(const std::vector<int>& ids,
                                sizeof(double) * (end - on_host),
                                hipMemcpyDeviceToHost, stream_),
                 "hipMemcpy(snapshots)");
-    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     on_host = end;
   };
   int64_t stop = nsteps;
@@ -925,7 +932,14 @@ const double* Engine::step_snapshots(const std::vector<int>& ids,
     std::sort(ends.begin(), ends.end());
     ends.erase(std::unique(ends.begin(), ends.end()), ends.end());
     const bool uniform = ends.size() == 1;
-    int64_t t0 = 0;
+    int64_t t0 = 0, pending = -1;  // the boundary whose deliver is due
+    auto flush = [&] {
+      if (pending < 0) return;
+      const int64_t t = pending;
+      pending = -1;
+      hip_check(hipEventSynchronize(ev_boundary_), "hipEventSynchronize");
+      if (deliver) deliver(t, total ? stage_[0].h : nullptr);
+    };
     for (int64_t e : ends) {
       if (e <= t0) continue;
       std::vector<int> active;
@@ -933,19 +947,25 @@ const double* Engine::step_snapshots(const std::vector<int>& ids,
         if ((*counts)[q] >= e) active.push_back(ids[q]);
       run_steps(active, e - t0, graph && uniform,
                 [&](int64_t t) { gather_after(t0 + t); });
+      flush();  // the previous boundary's host work, while this segment runs
       t0 = e;
       if (t0 < nsteps && boundary) {
-        to_host_before(t0);
-        if (boundary(t0, total ? stage_[0].h : nullptr)) {
+        copy_before(t0);
+        const bool cut = boundary(t0);
+        hip_check(hipEventRecord(ev_boundary_, stream_), "hipEventRecord");
+        pending = t0;
+        if (cut) {
           stop = t0;
           break;
         }
       }
     }
+    flush();
   }
   if (ran) *ran = stop;
   if (total == 0) return nullptr;
-  to_host_before(stop);
+  copy_before(stop);
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   return stage_[0].h;
 }
 
@@ -1038,11 +1058,25 @@ void Engine::download(int id, double* m, double* x, double* y, double* z,
 
 const double* Engine::gather(const std::vector<int>& ids, int nfields,
                              std::vector<int64_t>& rows, int64_t& total) {
+  const double* h = gather_async(ids, nfields, rows, total);
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  return h;
+}
+
+const double* Engine::gather_async(const std::vector<int>& ids, int nfields,
+                                   std::vector<int64_t>& rows,
+                                   int64_t& total) {
   if (nfields != 3 && nfields != 6)
     throw Error(NBODY_E_INVALID, "gather: nfields must be 3 or 6");
+  // the previous gather's list and host view are reused
+  if (gather_queued_) {
+    hip_check(hipEventSynchronize(ev_gather_), "hipEventSynchronize");
+    gather_queued_ = false;
+  }
   rows.clear();
   total = 0;
-  std::vector<WorkItem> items;
+  std::vector<WorkItem>& items = gather_items_;
+  items.clear();
   for (int id : ids) {
     check_id(id);
     rows.push_back(total);
@@ -1070,7 +1104,8 @@ const double* Engine::gather(const std::vector<int>& ids, int nfields,
   hip_check(hipMemcpyAsync(stage_[1].h, stage_[1].d, sizeof(double) * need,
                            hipMemcpyDeviceToHost, stream_),
             "hipMemcpy(gather)");
-  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  hip_check(hipEventRecord(ev_gather_, stream_), "hipEventRecord");
+  gather_queued_ = true;
   return stage_[1].h;
 }
 

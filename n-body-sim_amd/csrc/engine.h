@@ -97,17 +97,23 @@ class Engine {
   // distinct counts, each over the systems still stepping, so a short system
   // drops out without ending the call (graphs are used only when every
   // count is nsteps).  `boundary`, when given, is called at each boundary
-  // t0 < nsteps between segments, after the systems with count t0 have
-  // taken their last step, with the host view already holding every
-  // snapshot taken before step t0 (gather() may be called from it); when it
-  // returns true the call stops there: *ran receives the steps run (systems
-  // took min(count, *ran)) and snapshots at or after it hold no data.
-  // Synchronous.
+  // t0 < nsteps between segments, once the steps before t0 are queued (the
+  // systems with count t0 have taken their last step on the stream; nothing
+  // has been waited for).  It may queue device work of its own
+  // (gather_async) but must not read results.  When it returns true the
+  // call stops there: *ran receives the steps run (systems took
+  // min(count, *ran)) and snapshots at or after it hold no data.
+  // `deliver(t0, hv)` follows each boundary once the snapshots taken before
+  // step t0, and what the boundary queued, have reached the host (hv holds
+  // them; null when the call has no snapshots).  It runs after the next
+  // segment's launches are queued, so the host's handling overlaps the
+  // device's stepping, and always before the next boundary.  Synchronous.
   const double* step_snapshots(
       const std::vector<int>& ids, int64_t nsteps, bool graph,
       const std::vector<Snap>& snaps, std::vector<int64_t>& rows,
       int64_t& total, const std::vector<int64_t>* counts = nullptr,
-      const std::function<bool(int64_t, const double*)>& boundary = nullptr,
+      const std::function<bool(int64_t)>& boundary = nullptr,
+      const std::function<void(int64_t, const double*)>& deliver = nullptr,
       int64_t* ran = nullptr);
   // F(x) of one system into the device acceleration arrays (and optionally
   // to the host).  Drops a velocity-Verlet carry of that system.
@@ -124,6 +130,11 @@ class Engine {
   // base[f * total + rows[k]].  Synchronous.
   const double* gather(const std::vector<int>& ids, int nfields,
                        std::vector<int64_t>& rows, int64_t& total);
+  // gather() without the wait: the same view, valid once the stream has
+  // passed this point (synchronize(), or a deliver of step_snapshots() when
+  // called from its boundary).  A later gather first waits for this one.
+  const double* gather_async(const std::vector<int>& ids, int nfields,
+                             std::vector<int64_t>& rows, int64_t& total);
   void synchronize();
 
   // Graph-captured multi-step loop for a fixed set of systems (batched API).
@@ -234,6 +245,10 @@ class Engine {
   // [0]: step_snapshots(), [1]: gather(), so a gather between the segments
   // of a stepping call leaves that call's snapshots intact
   Staging stage_[2];
+  std::vector<WorkItem> gather_items_;  // host side of the last gather list
+  hipEvent_t ev_gather_ = nullptr;      // the last gather's copy is done
+  bool gather_queued_ = false;
+  hipEvent_t ev_boundary_ = nullptr;    // a boundary's copies are done
   WorkItem* d_snap_items_ = nullptr;  // gather lists of step_snapshots()
   size_t snap_items_cap_ = 0;
   double* d_traj_ = nullptr;    // trajectory buffer of record()

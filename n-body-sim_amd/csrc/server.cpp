@@ -383,28 +383,34 @@ class Server {
         std::lock_guard<std::mutex> lk(mu_);
         for (auto& e : evs) push_event(std::move(e));
       };
+      // requests that took their last step at a boundary: their final state
+      // is gathered there and reported, after the snapshots taken so far,
+      // while the next segment runs
+      Final fin;
       try {
         // a graph pays off only while the resident set stays the same; the
         // epoch is cut where a request ends if others wait for admission
         h = eng_->step_snapshots(
             ids, k, ids == prev_ids, snaps, rows, total, &counts,
-            [&](int64_t t0, const double* hv) {
-              // requests that just took their last step report at once,
-              // after the snapshots taken so far
-              emit_snaps(t0, hv);
-              std::vector<uint64_t> fin;
+            [&](int64_t) {
+              fin = Final{};
               {
                 std::lock_guard<std::mutex> lk(mu_);
                 for (auto& kv : active_) {
                   const Active& a = kv.second;
                   if (!a.done_sent && !a.cancel &&
                       eng_->steps_done(a.sys) >= a.req.steps)
-                    fin.push_back(kv.first);
+                    fin.ids.push_back(kv.first);
                 }
               }
-              emit_done(fin);
+              gather_final(fin, false);
               std::lock_guard<std::mutex> lk(mu_);
               return !queue_.empty();
+            },
+            [&](int64_t t0, const double* hv) {
+              emit_snaps(t0, hv);
+              emit_final(fin);
+              fin = Final{};
             },
             &ran);
       } catch (const std::exception& ex) {
@@ -479,27 +485,49 @@ class Server {
     for (auto& e : evs) push_event(std::move(e));
   }
 
-  // DONE (or FAILED) events of the listed finished requests, their final
-  // state gathered in one pass; marks them sent.
-  void emit_done(const std::vector<uint64_t>& fin) {
-    if (fin.empty()) return;
+  // Final states of finished requests, gathered in one pass.
+  struct Final {
+    std::vector<uint64_t> ids;
+    std::vector<int64_t> rows;
+    int64_t total = 0;
+    const double* h = nullptr;  // valid once the gather has completed
+    std::string err;
+  };
+
+  // Queues (wait = false) or runs the gather of fin's final states.
+  void gather_final(Final& fin, bool wait) {
+    if (fin.ids.empty()) return;
     std::vector<int> sys;
     {
       std::lock_guard<std::mutex> lk(mu_);
-      for (uint64_t id : fin) sys.push_back(active_.at(id).sys);
+      for (uint64_t id : fin.ids) sys.push_back(active_.at(id).sys);
     }
-    std::vector<int64_t> rows;
-    int64_t total = 0;
-    const double* h = nullptr;
-    std::string err;
     try {
-      h = eng_->gather(sys, 6, rows, total);
+      fin.h = wait ? eng_->gather(sys, 6, fin.rows, fin.total)
+                   : eng_->gather_async(sys, 6, fin.rows, fin.total);
     } catch (const std::exception& ex) {
-      err = ex.what();
+      fin.h = nullptr;
+      fin.err = ex.what();
     }
+  }
+
+  // DONE (or FAILED) events of the listed finished requests; marks them
+  // sent.
+  void emit_done(const std::vector<uint64_t>& ids) {
+    Final fin;
+    fin.ids = ids;
+    gather_final(fin, true);
+    emit_final(fin);
+  }
+
+  void emit_final(const Final& fin) {
+    if (fin.ids.empty()) return;
+    const double* h = fin.h;
+    const std::vector<int64_t>& rows = fin.rows;
+    const int64_t total = fin.total;
     std::lock_guard<std::mutex> lk(mu_);
-    for (size_t q = 0; q < fin.size(); ++q) {
-      Active& a = active_.at(fin[q]);
+    for (size_t q = 0; q < fin.ids.size(); ++q) {
+      Active& a = active_.at(fin.ids[q]);
       Event e;
       const int64_t done = eng_->steps_done(a.sys);
       if (h) {
@@ -510,7 +538,7 @@ class Server {
           f[c]->assign(h + c * total + rows[q], h + c * total + rows[q] + n);
         ++completed_;
       } else {
-        fill(e.ev, NBODY_EVENT_FAILED, a.req, done, a.t_admit, err.c_str());
+        fill(e.ev, NBODY_EVENT_FAILED, a.req, done, a.t_admit, fin.err.c_str());
         ++failed_;
       }
       a.done_sent = true;
