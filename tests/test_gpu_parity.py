@@ -357,3 +357,75 @@ def test_precision_switch_on_the_same_bodies(gpu):
         assert np.array_equal(r1[q], r2[q]), q
     assert rel_err(f64, ref).max() <= 1e-12
     assert rel_err(r1, ref).max() <= 1e-5
+
+
+def test_large_workload_full_size_sampled(gpu):
+    """N = 1,048,576 colliding spheres (softening 0.002), every precision,
+    against the oracle on 1024 sampled targets (all sources): the strict
+    form bit for bit, f64 within 1e-12, both f32 forms within 1e-5."""
+    n, eps = 1 << 20, 0.002
+    b = nb.colliding(n, 5, separation=10.0, rel_speed=0.5)
+    idx = np.sort(np.random.default_rng(21).choice(n, 1024, replace=False))
+    ref = oracle.newton_targets(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, eps, idx,
+                                min(16, oracle.default_threads()))
+    for prec, tol in ((nb.Precision.F64_STRICT, None), (nb.Precision.F64, 1e-12),
+                      (nb.Precision.F32C, 1e-5), (nb.Precision.F32R, 1e-5)):
+        got = tuple(a[idx] for a in gpu_acc(gpu, b, 1.0, eps, prec))
+        if tol is None:
+            for q in range(3):
+                assert np.array_equal(got[q], ref[q]), (prec.name, q)
+            print(f"N={n} {prec.name}: 1024 sampled targets bit-identical", flush=True)
+            continue
+        e = rel_err(got, ref)
+        print(f"N={n} {prec.name}: max {e.max():.3e} median {np.median(e):.3e}",
+              flush=True)
+        assert e.max() <= tol, prec.name
+
+
+def _batched_workload():
+    """The batched workload at its full shape: 512 star systems, N
+    log-uniform in [64, 8192], Euler / VV / RK4 in turn."""
+    rng = np.random.default_rng(3)
+    ns = np.exp(rng.uniform(np.log(64), np.log(8192), 512)).astype(np.int64)
+    systems = [nb.star_system(int(k), 10_000 + s) for s, k in enumerate(ns)]
+    return systems, [s % 3 for s in range(len(systems))]
+
+
+@pytest.mark.parametrize("prec", [nb.Precision.F32C, nb.Precision.F32R])
+def test_batched_workload_full_size_alone_vs_mixed(gpu, prec):
+    """Systems stepped inside the full 512-system batch end bit-identical to
+    the same systems stepped alone (largest, smallest and a middle one)."""
+    systems, integ = _batched_workload()
+    bt = nb.Batch(gpu, systems, integ, 1e-3, 1.0, 1e-6, prec)
+    bt.step(3)
+    mx = bt.download()
+    bt.close()
+    sizes = np.array([s.masses.shape[0] for s in systems])
+    picks = [int(np.argmax(sizes)), int(np.argmin(sizes)), int(np.argsort(sizes)[256])]
+    for s in picks:
+        alone = nb.Batch(gpu, [systems[s]], [integ[s]], 1e-3, 1.0, 1e-6, prec)
+        alone.step(3)
+        ax = alone.download()
+        alone.close()
+        o, k = bt.offsets[s], bt.n[s]
+        for q in range(6):
+            assert np.array_equal(ax[q], mx[q][o:o + k]), (prec.name, s, q)
+
+
+def test_batched_workload_full_size_strict_matches_oracle(gpu):
+    """The full batched workload in the strict form: two steps of every
+    system; the largest system of each integrator equals the oracle's."""
+    systems, integ = _batched_workload()
+    bt = nb.Batch(gpu, systems, integ, 1e-3, 1.0, 1e-6, nb.Precision.F64_STRICT)
+    bt.step(2)
+    out = bt.download()
+    bt.close()
+    sizes = [s.masses.shape[0] for s in systems]
+    for kind in range(3):
+        s = max((q for q in range(len(systems)) if integ[q] == kind),
+                key=lambda q: sizes[q])
+        ref = oracle.integrate(kind, *systems[s].arrays(), 1.0, 1e-3, 1e-3, 2,
+                               min(16, oracle.default_threads()))
+        o, k = bt.offsets[s], bt.n[s]
+        for q in range(6):
+            assert np.array_equal(out[q][o:o + k], ref[q]), (kind, s, q)
