@@ -211,6 +211,18 @@ bool Engine::fits(int64_t n) const {
   return false;
 }
 
+int64_t Engine::take_range(std::map<int64_t, int64_t>& free, int64_t n) {
+  const int64_t need = round_up(n, kAlign);
+  auto it = free.begin();
+  for (; it != free.end(); ++it)
+    if (it->second >= need) break;
+  if (it == free.end()) return -1;
+  const int64_t off = it->first, len = it->second;
+  free.erase(it);
+  if (len > need) free[off + need] = len - need;
+  return off;
+}
+
 int64_t Engine::free_bodies() const {
   int64_t t = 0;
   for (auto& f : free_) t += f.second;
@@ -247,13 +259,8 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
     }
   if (id < 0) throw Error(NBODY_E_NOSPACE, "system table full");
   const int64_t need = round_up(p.n, kAlign);
-  auto it = free_.begin();
-  for (; it != free_.end(); ++it)
-    if (it->second >= need) break;
-  if (it == free_.end()) throw Error(NBODY_E_NOSPACE, "arena full");
-  const int64_t off = it->first, len = it->second;
-  free_.erase(it);
-  if (len > need) free_[off + need] = len - need;
+  const int64_t off = take_range(free_, p.n);
+  if (off < 0) throw Error(NBODY_E_NOSPACE, "arena full");
 
   Sys& s = sys_[id];
   s = Sys{};

@@ -73,6 +73,13 @@ class Engine {
   void remove(int id);
   bool fits(int64_t n) const;
   int64_t free_bodies() const;
+  // The arena's free ranges (offset -> length), and the placement rule of
+  // add(): the first range, in offset order, that holds n bodies (rounded
+  // up to a wave).  take_range() carves that range from `free` and returns
+  // its offset, or -1 (leaving `free` as it was) when none holds n.
+  // Replaying it on a copy tells which of several uploads will fit.
+  std::map<int64_t, int64_t> free_ranges() const { return free_; }
+  static int64_t take_range(std::map<int64_t, int64_t>& free, int64_t n);
   int64_t capacity() const { return cap_; }
   int nsystems() const;
   const SystemParams& params(int id) const;

@@ -252,18 +252,17 @@ class Server {
     cv_events_.notify_all();
   }
 
-  // Admits queued requests that fit (worker thread, mu_ held).
+  // Admits queued requests that fit (worker thread, mu_ held).  Placement
+  // is replayed on a copy of the arena's free ranges with add()'s own rule,
+  // in admission order, so every admitted request gets a range when it is
+  // uploaded (ranges taken by earlier admissions of the same pass are gone).
   std::vector<Request> take_admissible() {
     std::vector<Request> out;
-    int64_t free_b = eng_->free_bodies();
+    std::map<int64_t, int64_t> free = eng_->free_ranges();
     int free_slots = cfg_.max_systems - eng_->nsystems();
     bool blocked = false;
     for (auto it = queue_.begin(); it != queue_.end() && free_slots > 0;) {
-      const int64_t need = (it->p.n + 63) / 64 * 64;
-      // contiguity is checked against the arena; pending admissions are
-      // accounted by the running free-body count
-      if (!blocked && need <= free_b && eng_->fits(it->p.n)) {
-        free_b -= need;
+      if (!blocked && Engine::take_range(free, it->p.n) >= 0) {
         --free_slots;
         out.push_back(std::move(*it));
         it = queue_.erase(it);

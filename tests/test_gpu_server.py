@@ -144,3 +144,82 @@ def test_cancel(gpu):
         srv.cancel(12345)
     assert e.value.code == nb.E_NOTFOUND
     srv.close()
+
+
+def served_workload(count, seed):
+    """Requests of the served workload's shape: Plummer or star systems with
+    N log-uniform in [32, 16384], 10 to 1000 steps, any integrator, dt 1e-3
+    or 5e-4."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(32), np.log(16384))))
+        b = nb.plummer(n, 9000 + k) if k % 2 == 0 else nb.star_system(n, 9000 + k)
+        out.append(dict(bodies=b, integrator=nb.IntegratorMethod(int(rng.integers(0, 3))),
+                        dt=float(rng.choice([1e-3, 5e-4])),
+                        steps=int(rng.integers(10, 1001)), eps2=1e-4))
+    return out
+
+
+@pytest.mark.parametrize("precision", [nb.Precision.F32C, nb.Precision.F32R])
+def test_served_workload_full_shape(gpu, precision):
+    """60 requests of the served workload submitted at once into a 131072-body
+    arena (admission waits for memory, epochs are cut at completions): every
+    snapshot arrives in order before its DONE, and the largest, the longest
+    and a middle request end bit-identical to the same request served alone."""
+    reqs = served_workload(60, 5)
+    ids, done, snaps, stats = run_server(reqs, precision, capacity=1 << 17)
+    assert stats["completed"] == len(reqs) and stats["failed"] == 0
+    for rid, r in zip(ids, reqs):
+        ev, _ = done[rid]
+        assert ev["kind"] == nb.EVENT_DONE and ev["step"] == r["steps"]
+        assert [s for s, _ in snaps[rid]] == list(range(10, r["steps"], 10))
+    sizes = [r["bodies"].masses.shape[0] for r in reqs]
+    picks = {int(np.argmax(sizes)), int(np.argmax([r["steps"] for r in reqs])), 30}
+    for k in sorted(picks):
+        aid, adone, asnaps, _ = run_server([reqs[k]], precision, capacity=1 << 17)
+        data_m, data_a = done[ids[k]][1], adone[aid[0]][1]
+        for q in range(6):
+            assert np.array_equal(data_m[q], data_a[q]), (k, q)
+        for (_, dm), (_, da) in zip(snaps[ids[k]], asnaps[aid[0]]):
+            for q in range(3):
+                assert np.array_equal(dm[q], da[q]), k
+
+
+def test_served_workload_strict_matches_oracle(gpu):
+    """The served workload's request shape in the strict form: 24 requests at
+    once; the largest and the smallest of those the oracle replays in
+    seconds (at most 3e10 pair terms) equal its trajectory bit for bit."""
+    reqs = served_workload(24, 6)
+    ids, done, _, stats = run_server(reqs, nb.Precision.F64_STRICT, snapshots=False,
+                                     capacity=1 << 17)
+    assert stats["completed"] == len(reqs)
+    sizes = [r["bodies"].masses.shape[0] for r in reqs]
+    cheap = [k for k, r in enumerate(reqs) if sizes[k] ** 2 * r["steps"] * 4 <= 3e10]
+    for k in {max(cheap, key=lambda q: sizes[q]), min(cheap, key=lambda q: sizes[q])}:
+        r = reqs[k]
+        ref = oracle.integrate(int(r["integrator"]), *r["bodies"].arrays(), 1.0, 0.01,
+                               r["dt"], r["steps"], min(16, oracle.default_threads()))
+        data = done[ids[k]][1]
+        for q in range(6):
+            assert np.array_equal(data[q], ref[q]), (k, q)
+
+
+def test_admission_replays_placement(gpu):
+    """Admission must place a pass's requests as the upload will.  A 640-body
+    arena takes 384 + 64 + 192 bodies; the 384 and the 192 finish first while
+    the 64 runs on, leaving free ranges of 384 and 192.  Of the waiting
+    requests, 256 takes the front of the 384 range; 320 then fits in no range
+    (though 320 bodies are free in total and it fit before the 256 was
+    placed), so it must wait for the 256 to finish, not fail."""
+    def req(n, steps, seed):
+        return dict(bodies=nb.plummer(n, seed), integrator=nb.IntegratorMethod.EULER,
+                    dt=1e-3, steps=steps, eps2=1e-4)
+    reqs = [req(384, 10, 1), req(64, 200, 2), req(192, 10, 3), req(256, 10, 4),
+            req(320, 10, 5)]
+    ids, done, _, stats = run_server(reqs, nb.Precision.F32C, snapshots=False,
+                                     capacity=640)
+    for rid, r in zip(ids, reqs):
+        ev, _ = done[rid]
+        assert ev["kind"] == nb.EVENT_DONE, (r["bodies"].masses.shape[0], ev["message"])
+    assert stats["completed"] == len(reqs) and stats["failed"] == 0
