@@ -223,3 +223,71 @@ def test_admission_replays_placement(gpu):
         ev, _ = done[rid]
         assert ev["kind"] == nb.EVENT_DONE, (r["bodies"].masses.shape[0], ev["message"])
     assert stats["completed"] == len(reqs) and stats["failed"] == 0
+
+
+def test_served_workload_cancel_under_backpressure(gpu):
+    """40 requests of the served workload's shape with a 4 MB cap on queued
+    snapshot bytes (the worker pauses until the client drains) and every
+    fifth request cancelled, half at once and half later.  Each request ends
+    with one final event and no snapshot after it; every finished request has
+    all its snapshots in order; uncancelled requests equal their runs alone."""
+    reqs = served_workload(40, 7)
+    srv = nb.Server(capacity_bodies=1 << 17, max_systems=256, snapshot_every=10,
+                    precision=nb.Precision.F32C, max_snapshot_bytes=4 << 20)
+    ids = [srv.submit(r["bodies"], r["integrator"], r["dt"], r["steps"],
+                      eps2=r["eps2"], want_snapshots=True) for r in reqs]
+    cancel_now = ids[0::10]
+    cancel_later = ids[5::10]
+
+    def cancel(rid):   # a request already retired is no longer known
+        try:
+            srv.cancel(rid)
+        except nb.NbodyError as e:
+            assert e.code == nb.E_NOTFOUND
+
+    for rid in cancel_now:
+        cancel(rid)
+    final, snaps = {}, {rid: [] for rid in ids}
+    polled = 0
+    t_end = time.time() + 240
+    while len(final) < len(ids):
+        assert time.time() < t_end, "server did not finish in time"
+        if polled < 40:
+            time.sleep(0.01)   # a slow reader first: the snapshot cap fills
+        r = srv.poll(timeout_ms=1000, capacity=1 << 16)
+        if r is None:
+            continue
+        polled += 1
+        if polled == 60:
+            for rid in cancel_later:
+                if rid not in final:
+                    cancel(rid)
+        ev, data = r
+        rid = ev["id"]
+        assert rid not in final, "event after the request's final event"
+        if ev["kind"] == nb.EVENT_SNAPSHOT:
+            snaps[rid].append((ev["step"], data))
+        else:
+            final[rid] = (ev, data)
+    stats = srv.stats()
+    srv.close()
+    assert stats["failed"] == 0
+    cancelled = set(cancel_now) | set(cancel_later)
+    for rid, r in zip(ids, reqs):
+        ev, _ = final[rid]
+        steps = [s for s, _ in snaps[rid]]
+        assert steps == list(range(10, 10 * len(steps) + 1, 10))[:len(steps)]
+        if ev["kind"] == nb.EVENT_DONE:
+            assert ev["step"] == r["steps"]
+            assert steps == list(range(10, r["steps"], 10))
+        else:
+            assert ev["kind"] == nb.EVENT_CANCELLED and rid in cancelled
+    for k in (1, 13, 27):      # never cancelled
+        assert final[ids[k]][0]["kind"] == nb.EVENT_DONE
+        aid, adone, asnaps, _ = run_server([reqs[k]], nb.Precision.F32C,
+                                           capacity=1 << 17)
+        for q in range(6):
+            assert np.array_equal(final[ids[k]][1][q], adone[aid[0]][1][q]), (k, q)
+        for (_, dm), (_, da) in zip(snaps[ids[k]], asnaps[aid[0]]):
+            for q in range(3):
+                assert np.array_equal(dm[q], da[q]), k
