@@ -11,6 +11,8 @@
   terms within 1e-12;
 * 300k bodies: the theta error law against the GPU direct sum, a property
   that holds at any size;
+* 1,048,576 colliding-sphere bodies: strict bit for bit and fast within
+  1e-12 of the oracle on sampled targets;
 * the command line with --gravity barnes-hut writes the oracle's file.
 """
 import os
@@ -190,3 +192,24 @@ def test_cli_barnes_hut_bytes(gpu, tmp_path, name):
     want = np.stack([np.vstack([b.pos_x[None], xs]), np.vstack([b.pos_y[None], ys]),
                      np.vstack([b.pos_z[None], zs])], -1)
     assert np.array_equal(pos, want)
+
+
+def test_bh_full_size_sampled(gpu):
+    """N = 1,048,576 colliding spheres (the large workload), theta 0.5,
+    against the oracle's Barnes-Hut on 1024 sampled targets (full tree):
+    strict bit for bit, fast f64 within 1e-12."""
+    n, eps, theta = 1 << 20, 0.002, 0.5
+    b = nb.colliding(n, 5, separation=10.0, rel_speed=0.5)
+    idx = np.sort(np.random.default_rng(23).choice(n, 1024, replace=False))
+    ref = oracle.barnes_hut_targets(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, eps,
+                                    theta, idx, min(16, oracle.default_threads()))
+    acc, nodes = gpu_bh(gpu, b, 1.0, eps, theta)
+    print(f"N={n}: {nodes} nodes", flush=True)
+    for q in range(3):
+        assert np.array_equal(acc[q][idx], ref[q]), q
+    fast, _ = gpu_bh(gpu, b, 1.0, eps, theta, nb.Precision.F64)
+    got = np.stack([a[idx] for a in fast], 1)
+    r = np.stack(ref, 1)
+    e = np.linalg.norm(got - r, axis=1) / np.linalg.norm(r, axis=1)
+    print(f"N={n} fast: max {e.max():.3e} median {np.median(e):.3e}", flush=True)
+    assert e.max() <= 1e-12
