@@ -6,17 +6,18 @@
 // may be bypassed by later ones at most kMaxBypass times before admission
 // stops behind it (no starvation).  All resident systems advance together in
 // epochs of snapshot_every steps; a system whose request ends inside an
-// epoch stops stepping at its last step while the others go on, and is
-// retired at the epoch's end; the epoch is cut at such a step when other
-// requests wait for admission.  Snapshots are taken at multiples of
-// snapshot_every of each system's own steps, by gather launches between the
-// epoch's steps,
-// so every system is snapshotted and retired at the same own-step counts
-// whether it runs alone or in a mix (its target reordering in MODE_F32R also
-// follows its own step count).  New requests join at the next epoch.  The
-// epoch's snapshots leave the device in one copy and are queued as events;
-// when the queued snapshot bytes exceed the configured cap the worker
-// pauses stepping until the client drains events (backpressure).
+// epoch stops stepping at its last step while the others go on, reports
+// DONE at that point, and is retired at the epoch's end; the epoch is cut at
+// such a step when other requests wait for admission.  Snapshots are taken
+// at multiples of snapshot_every of each system's own steps, by gather
+// launches between the epoch's steps, so every system is snapshotted and
+// retired at the same own-step counts whether it runs alone or in a mix (its
+// target reordering in MODE_F32R also follows its own step count).  New
+// requests join at the next epoch.  Snapshots taken before a request's end
+// leave the device with its final state and are queued as events while the
+// next segment runs; the rest leave at the epoch's end.  When the queued
+// snapshot bytes exceed the configured cap the worker pauses stepping until
+// the client drains events (backpressure).
 #include "server.h"
 
 #include <hip/hip_runtime_api.h>
