@@ -100,6 +100,7 @@ def test_examples_all_precisions(gpu, name):
         assert np.array_equal(got[q], ref[q])
     check_small(gpu_acc(gpu, b, g, eps, nb.Precision.F64), ref, b, g, eps, 1e-12)
     check_small(gpu_acc(gpu, b, g, eps, nb.Precision.F32C), ref, b, g, eps, 1e-5)
+    check_small(gpu_acc(gpu, b, g, eps, nb.Precision.F32R), ref, b, g, eps, 1e-5)
 
 
 @pytest.mark.parametrize("n", [1000, 4096])
@@ -192,12 +193,13 @@ def test_examples_trajectories_strict(gpu, name, integ):
 
 
 @pytest.mark.parametrize("integ", INTEG)
-def test_f64_and_f32c_trajectories(gpu, integ):
+def test_f64_and_f32_trajectories(gpu, integ):
     n, g, eps, dt, k = 512, 1.0, 0.01, 1e-3, 20
     b = nb.plummer(n, 55)
     ref = oracle.integrate(int(integ), b.masses, b.pos_x, b.pos_y, b.pos_z,
                            b.vel_x, b.vel_y, b.vel_z, g, eps, dt, k)
-    for prec, tol in ((nb.Precision.F64, 1e-12), (nb.Precision.F32C, 1e-7)):
+    for prec, tol in ((nb.Precision.F64, 1e-12), (nb.Precision.F32C, 1e-7),
+                      (nb.Precision.F32R, 1e-7)):
         db = nb.DeviceBodies(gpu, b)
         integ.gpu_create(dt).step(gpu, db, nb.GpuNewtonParallelGravity(g, eps * eps, prec), k)
         out = db.download()
