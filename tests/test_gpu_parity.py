@@ -447,3 +447,38 @@ def test_unsoftened_star_system_all_forms(gpu):
         print(f"eps=0 {prec.name}: max {e.max():.3e} median {np.median(e):.3e}",
               flush=True)
         check_small(got, ref, b, 1.0, 0.0, tol)
+
+
+def test_coincident_bodies_unsoftened(gpu):
+    """Two distinct bodies at one point, eps = 0.  F64_STRICT, F64 and F32C
+    follow the reference (1/sqrt(0) = inf, inf * 0 = NaN): the pair's two
+    bodies get NaN; the others get the oracle's bits (strict) or agree within
+    the form's tolerance.  F32R drops pairs at r2 == 0 (that is how it
+    excludes a target's own term), so every body gets the finite sum over
+    its other sources."""
+    m = np.array([1.0, 2.0, 3.0, 4.0, 5.0])
+    x = np.array([0.0, 0.0, 1.0, 0.0, 0.0])
+    y = np.array([0.0, 0.0, 0.0, 2.0, 0.0])
+    z = np.array([0.0, 0.0, 0.0, 0.0, 3.0])
+    b = nb.Bodies(m, x, y, z, np.zeros(5), np.zeros(5), np.zeros(5))
+    ref = oracle.newton(m, x, y, z, 1.0, 0.0)
+    assert np.isnan(ref[0][:2]).all() and np.isfinite(ref[0][2:]).all()
+    got = gpu_acc(gpu, b, 1.0, 0.0, nb.Precision.F64_STRICT)
+    for q in range(3):
+        assert np.array_equal(got[q], ref[q], equal_nan=True), q
+    # the masked sum, in f64 with numpy
+    p = np.stack([x, y, z], 1)
+    d = p[None] - p[:, None]
+    r2 = (d * d).sum(-1)
+    w = np.where(r2 > 0, m[None] / np.where(r2 > 0, r2, 1.0) ** 1.5, 0.0)
+    want = tuple((w * d[..., q]).sum(1) for q in range(3))
+    for prec, tol in ((nb.Precision.F64, 1e-12), (nb.Precision.F32C, 1e-5)):
+        got = gpu_acc(gpu, b, 1.0, 0.0, prec)
+        for q in range(3):
+            assert np.isnan(got[q][:2]).all(), (prec.name, q)
+        e = rel_err(tuple(a[2:] for a in got), tuple(a[2:] for a in ref))
+        assert e.max() <= tol, (prec.name, e.max())
+    got = gpu_acc(gpu, b, 1.0, 0.0, nb.Precision.F32R)
+    assert all(np.isfinite(a).all() for a in got)
+    e = rel_err(got, want)
+    assert e.max() <= 1e-5, e.max()
