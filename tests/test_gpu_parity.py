@@ -482,3 +482,20 @@ def test_coincident_bodies_unsoftened(gpu):
     assert all(np.isfinite(a).all() for a in got)
     e = rel_err(got, want)
     assert e.max() <= 1e-5, e.max()
+
+
+@pytest.mark.parametrize("shift", [1e4, -3e6])
+def test_system_far_from_the_origin(gpu, shift):
+    """A 4096-body Plummer sphere moved far from the coordinate origin: the
+    forms difference positions in f64 (F32R against each wave's own origin),
+    so every form keeps its tolerance against the oracle on the same moved
+    coordinates."""
+    b = nb.plummer(4096, 31)
+    b = nb.Bodies(b.masses, b.pos_x + shift, b.pos_y - 2 * shift, b.pos_z + 0.5 * shift,
+                  b.vel_x, b.vel_y, b.vel_z)
+    ref = oracle.newton_parallel(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, 0.005)
+    for prec, tol in ((nb.Precision.F64, 1e-12), (nb.Precision.F32C, 1e-5),
+                      (nb.Precision.F32R, 1e-5)):
+        e = rel_err(gpu_acc(gpu, b, 1.0, 0.005, prec), ref)
+        print(f"shift {shift:g} {prec.name}: max {e.max():.3e}", flush=True)
+        assert e.max() <= tol, prec.name
