@@ -204,13 +204,6 @@ int64_t Engine::steps_done(int id) const {
   return sys_[id].steps;
 }
 
-bool Engine::fits(int64_t n) const {
-  const int64_t need = round_up(n, kAlign);
-  for (auto& f : free_)
-    if (f.second >= need) return true;
-  return false;
-}
-
 int64_t Engine::take_range(std::map<int64_t, int64_t>& free, int64_t n) {
   const int64_t need = round_up(n, kAlign);
   auto it = free.begin();
@@ -221,12 +214,6 @@ int64_t Engine::take_range(std::map<int64_t, int64_t>& free, int64_t n) {
   free.erase(it);
   if (len > need) free[off + need] = len - need;
   return off;
-}
-
-int64_t Engine::free_bodies() const {
-  int64_t t = 0;
-  for (auto& f : free_) t += f.second;
-  return t;
 }
 
 static void validate(const SystemParams& p) {

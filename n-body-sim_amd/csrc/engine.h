@@ -71,8 +71,6 @@ class Engine {
           const double* y, const double* z, const double* vx,
           const double* vy, const double* vz);
   void remove(int id);
-  bool fits(int64_t n) const;
-  int64_t free_bodies() const;
   // The arena's free ranges (offset -> length), and the placement rule of
   // add(): the first range, in offset order, that holds n bodies (rounded
   // up to a wave).  take_range() carves that range from `free` and returns
