@@ -1103,10 +1103,6 @@ const double* Engine::gather_async(const std::vector<int>& ids, int nfields,
   return stage_[1].h;
 }
 
-void Engine::synchronize() {
-  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-}
-
 // The step loop of step() with one gather launch after every step.  The
 // gather of step s (s counted within the current buffer fill) writes field
 // f at d_traj_[f * C * n + s * n + i], C = steps per fill, so each field of

@@ -47,7 +47,6 @@ class Engine {
   Engine(const Engine&) = delete;
   Engine& operator=(const Engine&) = delete;
 
-  int mode() const { return mode_; }
   // Changing the arithmetic rebuilds every snapshot and drops carried state.
   void set_mode(int mode);
   // Force law (GRAV_*) and Barnes-Hut opening angle.  A change drops
@@ -55,7 +54,6 @@ class Engine {
   // Barnes-Hut evaluates in f64: the strict arithmetic in MODE_F64_STRICT,
   // the fast f64 arithmetic in every other mode.
   void set_gravity(int gravity, double theta);
-  int gravity() const { return gravity_; }
   // Largest source-range split given to systems uploaded afterwards (the
   // tuning table's per-size choice is clamped to it).  A system keeps its
   // split for life, so its summation order never depends on what shares its
@@ -136,11 +134,11 @@ class Engine {
   const double* gather(const std::vector<int>& ids, int nfields,
                        std::vector<int64_t>& rows, int64_t& total);
   // gather() without the wait: the same view, valid once the stream has
-  // passed this point (synchronize(), or a deliver of step_snapshots() when
-  // called from its boundary).  A later gather first waits for this one.
+  // passed this point (a deliver of step_snapshots() when called from its
+  // boundary, or any synchronous call).  A later gather first waits for
+  // this one.
   const double* gather_async(const std::vector<int>& ids, int nfields,
                              std::vector<int64_t>& rows, int64_t& total);
-  void synchronize();
 
   // Graph-captured multi-step loop for a fixed set of systems (batched API).
   void step_graph(const std::vector<int>& ids, int64_t nsteps);
@@ -153,8 +151,6 @@ class Engine {
   void record(int id, int64_t nsteps, double* x, double* y, double* z,
               size_t max_bytes);
 
-  const Arena& arena() const { return A_; }
-  hipStream_t stream() const { return stream_; }
 
  private:
   struct Sys {
