@@ -1,30 +1,44 @@
 """Benchmark: pair-interactions/s of the direct-sum gravity + integrator step.
 
-Default workload (one GPU): a single N=65536 Plummer sphere (seed 2, scale
-radius 1, equal masses 1/N, G=1), softening 0.005, dt=5e-4, velocity Verlet
-with the acceleration carried between steps (one force evaluation per step).
-The headline `value` is an f32-compensated form (--f32: f32r, per-wave
-relative coordinates, the default for one large system; or f32c, one f64
-difference per pair, the default for the batched and served mixes of small
-systems); the other f32 form and the plain f64 form are timed in the same run
-and reported beside it.  A "step" is one velocity-Verlet step
-of the whole system; interactions counted as N(N-1) per force evaluation and
-20 flops per interaction.  Inputs are resident in HBM before timing starts;
-the timed region is bracketed by HIP events on the library's stream.
+The default run (`python bench.py`, one GPU) prints ONE JSON line: the
+single-system workload below as the headline record, with the batched, served
+and large workloads as sub-records (`batched`, `serving`, `large`) of the same
+line.  Every workload is timed in an f32-compensated form (the headline) and
+in the plain f64 form (`fp64_plain`, with the f32/f64 ratio), and carries its
+accuracy against the f64 oracle.
 
-Other workloads: --workload batched (512 star systems, N log-uniform in
-[64, 8192], Euler/VV/RK4 1:1:1, dt=1e-3), --workload large (two colliding
-Plummer spheres, N=1,048,576, RK4), --workload serve (the request server
-under Poisson arrivals) and --workload bh (Barnes-Hut, theta 0.5, on the
-N=1,048,576 system).
+* single (headline `value`): one N=65536 Plummer sphere (seed 2, scale radius
+  1, equal masses 1/N, G=1), softening 0.005, dt=5e-4, velocity Verlet with
+  the acceleration carried between steps (one force evaluation per step),
+  `--steps` timed steps after `--warmup`.  Form f32r (per-wave relative
+  coordinates); f32c is timed beside it.
+* batched: 512 star systems, N log-uniform in [64, 8192], Euler/VV/RK4
+  1:1:1, dt=1e-3, softening 1e-3, 500 steps (`--batched-steps`), graph
+  stepping.
+* serving: the request server under Poisson arrivals at several rates
+  (`--rates`, 200 requests each): N log-uniform in [32, 16384], 10-1000
+  steps, random integrator, dt in {1e-3, 5e-4}, snapshots every 10 steps;
+  solves/s, p50/p99 latency and interactions/s per rate, and the saturated
+  rate again with the f64 server.
+* large: two colliding Plummer spheres, N=1,048,576, softening 0.002, RK4,
+  dt=2.5e-4, 20 steps (`--large-steps`; the second f32 form runs 5).
+
+A "step" of the headline is one velocity-Verlet step of the whole system;
+interactions are N(N-1) per force evaluation, 20 flops each.  Inputs are
+resident in HBM before timing starts; each timed region is bracketed by HIP
+events on the library's stream.  `roofline` is the force kernel's: its
+launches inside the timed region are bracketed by HIP events, and achieved
+FLOP/s = 20 x interactions per launch / mean launch time.
 
 cpu_baseline: the oracle's threaded direct sum (the reference's
-newton-parallel restated in C) on this host, full N, repeated for ~10 s.
+newton-parallel restated in C, bit-identical per target) on this host, full
+N, repeated for about --cpu-seconds.
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -36,22 +50,39 @@ import nbody_amd as nb  # noqa: E402
 
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}  # MI355X vector peaks (datasheet)
 FLOPS_PER_INTERACTION = 20
+# one GPU's share of a box's host cores: worker pools are capped at this
+HOST_CPU_SHARE = 16
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def rel_err(a, ref):
+def rel_err_vec(a, ref):
     a, ref = np.stack(a, 1), np.stack(ref, 1)
-    e = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    return np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+
+
+def rel_err(a, ref):
+    e = rel_err_vec(a, ref)
     return float(e.max()), float(np.median(e))
+
+
+def host_cpus():
+    """Threads for the CPU baseline (the reference's rayon pool spans every
+    logical CPU the process may run on, src/gravity/cpu/newton_parallel.rs:
+    44-51) capped at one GPU's share of the box, and the counts seen."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        aff = os.cpu_count() or 1
+    return min(aff, HOST_CPU_SHARE), aff, os.cpu_count()
 
 
 def cpu_baseline(b, g, eps, seconds):
     """Oracle newton-parallel on the host: full-N evaluations for ~seconds."""
     import oracle
-    threads = min(16, oracle.default_threads())
+    threads, aff, logical = host_cpus()
     n = len(b)
     t0 = time.perf_counter()
     reps = 0
@@ -78,8 +109,11 @@ def cpu_baseline(b, g, eps, seconds):
     per_eval["1048576 (extrapolated)"] = 1e3 * (1 << 20) * ((1 << 20) - 1) / rate
     return {"value": rate, "unit": "pair-interactions/s", "cores": threads,
             "kind": "port",
+            "host_logical_cpus": logical, "affinity_cpus": aff,
             "sample": f"{reps} full force evaluation(s) of the same N={n} system "
-                      f"(oracle newton-parallel, {threads} threads, {dt:.1f} s)",
+                      f"(oracle newton-parallel, {threads} threads = the "
+                      f"{aff} CPUs this process may use, capped at one GPU's "
+                      f"share of {HOST_CPU_SHARE}; {dt:.1f} s)",
             "ms_per_eval": per_eval}, acc
 
 
@@ -96,24 +130,6 @@ def pair_scale(b, g, eps, idx):
         w[np.arange(len(sel)), sel] = 0.0
         out[k:k + 32] = w.sum(1)
     return out
-
-
-def time_single(gpu, b, prec, g, eps, dt, steps, warmup):
-    grav = nb.GpuNewtonParallelGravity(g, eps * eps, prec)
-    integ = nb.GpuVelocityVerletIntegrator(dt)
-    db = nb.DeviceBodies(gpu, b)
-    acc0 = grav.calculate_accelerations(gpu, db)
-    e0 = energy(gpu, db, g, eps)
-    integ.step(gpu, db, grav, warmup)
-    gpu.synchronize()
-    gpu.record(0)
-    integ.step(gpu, db, grav, steps)
-    gpu.record(1)
-    ms = gpu.elapsed_ms(0, 1)
-    e1 = energy(gpu, db, g, eps)
-    final = db.download()
-    db.close()
-    return ms, acc0, e0, e1, final
 
 
 def energy(gpu, db, g, eps):
@@ -138,53 +154,86 @@ def measured_traffic():
     return None
 
 
-def roofline(rate, dtype, kernel_ms, traffic=None):
-    """VALU roofline of the force kernel: achieved = interactions/s x 20 flops
-    (the counting convention of the reference's published figures) against
-    MI355X's vector peak for the arithmetic type."""
-    ach = rate * FLOPS_PER_INTERACTION / 1e12
-    return {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS[dtype],
-            "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dtype], "traffic": traffic,
-            "kernel_avg_ms": kernel_ms}
+def roofline(dtype, interactions, kt, step_rate=None, traffic=None):
+    """VALU roofline of the force kernel.  kt: launch times of the timed
+    region (GpuContext.kernel_times()); interactions: pair interactions those
+    force launches evaluated.  achieved = 20 x interactions / force-launch
+    time (the counting convention of the reference's published figures)
+    against MI355X's vector peak for the arithmetic type."""
+    force_s = kt["force_ms"] / 1e3
+    ach = interactions * FLOPS_PER_INTERACTION / force_s / 1e12
+    r = {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS[dtype],
+         "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dtype], "traffic": traffic,
+         "basis": "force-kernel launches bracketed by HIP events in the timed region",
+         "force_kernel_avg_ms": kt["force_ms"] / max(kt["force_launches"], 1),
+         "force_launches": kt["force_launches"],
+         "combine_kernel_avg_ms": (kt["combine_ms"] / kt["combine_launches"]
+                                   if kt["combine_launches"] else 0.0),
+         "combine_launches": kt["combine_launches"]}
+    if step_rate is not None:
+        r["frac_of_whole_step"] = step_rate * FLOPS_PER_INTERACTION / 1e12 / PEAK_TFLOPS[dtype]
+    return r
 
 
 F32_FORMS = {"f32r": nb.Precision.F32R, "f32c": nb.Precision.F32C}
+PRECS = dict(F32_FORMS, f64=nb.Precision.F64)
 DTYPE = {
     "f32r": "f32 pair terms (per-wave relative coordinates; f64 differences "
             "split in two f32 words for near chunks), f64 state",
     "f32c": "f32 pair terms (f64 differences), f64 state",
+    "f64": "f64",
 }
 
 
-def default_f32(workload):
-    """The faster f32 form for a workload: many small systems (a few chunks
-    each, most of them near their targets) favour the per-pair f64 difference
-    of f32c; one large system favours f32r."""
-    return "f32c" if workload in ("batched", "serve") else "f32r"
+def other_f32(form):
+    return "f32c" if form == "f32r" else "f32r"
 
 
-def f32_forms(args):
-    """The headline f32 form first, then the other one."""
-    other = "f32c" if args.f32 == "f32r" else "f32r"
-    return [args.f32, other]
+# ---- config 2: one N=65536 Plummer sphere, velocity Verlet -------------------
+def time_single(gpu, b, prec, g, eps, dt, steps, warmup):
+    grav = nb.GpuNewtonParallelGravity(g, eps * eps, prec)
+    integ = nb.GpuVelocityVerletIntegrator(dt)
+    db = nb.DeviceBodies(gpu, b)
+    acc0 = grav.calculate_accelerations(gpu, db)
+    e0 = energy(gpu, db, g, eps)
+    integ.step(gpu, db, grav, warmup)
+    gpu.synchronize()
+    gpu.kernel_timing(True)
+    gpu.record(0)
+    integ.step(gpu, db, grav, steps)
+    gpu.record(1)
+    ms = gpu.elapsed_ms(0, 1)
+    kt = gpu.kernel_times()
+    gpu.kernel_timing(False)
+    e1 = energy(gpu, db, g, eps)
+    final = db.download()
+    db.close()
+    return ms, kt, acc0, e0, e1, final
 
 
 def run_single(args, gpu):
     n, g, eps, dt = 65536, 1.0, 0.005, 5e-4
     b = nb.plummer(n, 2)
-    inter = n * (n - 1) * args.steps
-    forms = f32_forms(args) + ["f64"]
-    precs = dict(F32_FORMS, f64=nb.Precision.F64)
+    inter_eval = n * (n - 1)
+    inter = inter_eval * args.steps
+    head = args.f32 or "f32r"
+    alt = other_f32(head)
+    forms = [head, alt, "f64"]
     res = {}
     for name in forms:
-        prec = precs[name]
-        log(f"timing {prec.name}: N={n} VV {args.steps} steps (+{args.warmup} warmup)")
-        res[name] = time_single(gpu, b, prec, g, eps, dt, args.steps, args.warmup)
-        log(f"  {res[name][0]:.1f} ms -> {inter / (res[name][0] / 1e3):.4e} int/s")
+        log(f"single {name}: N={n} VV {args.steps} steps (+{args.warmup} warmup)")
+        res[name] = time_single(gpu, b, PRECS[name], g, eps, dt, args.steps, args.warmup)
+        log(f"  {res[name][0]:.1f} ms -> {inter / (res[name][0] / 1e3):.4e} int/s; "
+            f"force kernel {res[name][1]['force_ms'] / max(res[name][1]['force_launches'], 1):.3f} ms")
     traffic = measured_traffic() or {}
-    head, alt = forms[0], forms[1]
     rate = {k: inter / (res[k][0] / 1e3) for k in forms}
     ms = {k: res[k][0] for k in forms}
+
+    def roof(k):
+        kt = res[k][1]
+        return roofline("f64" if k == "f64" else "f32", inter_eval * kt["force_launches"],
+                        kt, rate[k], traffic.get(k))
+
     cpu = None
     acc_ref = None
     if not args.no_cpu_baseline:
@@ -195,14 +244,14 @@ def run_single(args, gpu):
     # momentum drift |P(t) - P(0)| / sum_i m_i |v_i(0)|
     pscale = float(np.sum(b.masses * np.sqrt(b.vel_x ** 2 + b.vel_y ** 2 + b.vel_z ** 2)))
     pdrift = lambda e0, e1: float(np.linalg.norm(e1[3:6] - e0[3:6])) / pscale  # noqa: E731
-    fin64 = res["f64"][4]
+    fin64 = res["f64"][5]
     if acc_ref is not None:
         # normalised error |da_i| / sum_j |a_ij| on a seeded target sample
         sample = np.sort(np.random.default_rng(5).choice(n, 2048, replace=False))
         scale = pair_scale(b, g, eps, sample)
         ref_s = np.stack(acc_ref, 1)[sample]
     for k in forms:
-        _, acc0, e0, e1, fin = res[k]
+        _, _, acc0, e0, e1, fin = res[k]
         if acc_ref is not None:
             acc_rep[f"{k}_max_rel"], acc_rep[f"{k}_median_rel"] = rel_err(acc0, acc_ref)
             dn = np.linalg.norm(np.stack(acc0, 1)[sample] - ref_s, axis=1) / scale
@@ -218,6 +267,7 @@ def run_single(args, gpu):
             acc_rep[f"trajectory_max_dev_{k}_vs_f64"] = float(dev.max())
             acc_rep[f"trajectory_rms_dev_{k}_vs_f64"] = float(np.sqrt(np.mean(dev ** 2)))
     acc_rep["trajectory_steps"] = args.warmup + args.steps
+    acc_rep["reference"] = "oracle newton-parallel (f64, bit-identical to the reference's newton)"
     return {
         "metric": "pair-interactions/s (f32-compensated forces, f64 state)",
         "value": rate[head], "unit": "pair-interactions/s", "n_gpus": 1,
@@ -230,204 +280,342 @@ def run_single(args, gpu):
                    "softening": eps, "dt": dt, "integrator": "velocity-verlet",
                    "f32_form": head, "global_batch": 1,
                    "parallelism": "none (1 GPU)"},
-        "roofline": roofline(rate[head], "f32", ms[head] / args.steps,
-                             traffic.get(head)),
+        "roofline": roof(head),
         "cpu_baseline": cpu,
         "f32_alt": {"form": alt, "value": rate[alt], "dtype": DTYPE[alt],
-                    "ms_per_step": ms[alt] / args.steps,
-                    "roofline": roofline(rate[alt], "f32", ms[alt] / args.steps,
-                                         traffic.get(alt))},
+                    "ms_per_step": ms[alt] / args.steps, "roofline": roof(alt)},
         "fp64_plain": {"value": rate["f64"], "ms_per_step": ms["f64"] / args.steps,
-                       "roofline": roofline(rate["f64"], "f64", ms["f64"] / args.steps,
-                                            traffic.get("f64")),
-                       "f32_speedup": rate[head] / rate["f64"]},
+                       "roofline": roof("f64"), "f32_speedup": rate[head] / rate["f64"]},
         "accuracy": acc_rep,
     }
 
 
-def run_batched(args, gpu):
+# ---- config 3: 512 star systems, Euler / VV / RK4 ---------------------------
+BATCH_EPS = 1e-3
+
+
+def batched_systems():
     rng = np.random.default_rng(3)
     nsys = 512
     ns = np.exp(rng.uniform(np.log(64), np.log(8192), nsys)).astype(np.int64)
     systems = [nb.star_system(int(n), 10_000 + s) for s, n in enumerate(ns)]
     integ = [s % 3 for s in range(nsys)]
+    return ns, systems, integ
+
+
+def batch_force_accuracy(gpu, systems, ns, forms):
+    """Max / median relative acceleration error against the oracle on 12
+    systems spanning the size range (the smallest, the largest and ten
+    quantiles), each evaluated alone in every form."""
+    import oracle
+    order = np.argsort(ns)
+    pick = sorted(set(order[np.linspace(0, len(ns) - 1, 12).astype(int)].tolist()))
+    errs = {k: [] for k in forms}
+    for s in pick:
+        b = systems[s]
+        ref = oracle.newton_parallel(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, BATCH_EPS,
+                                     host_cpus()[0])
+        for k in forms:
+            db = nb.DeviceBodies(gpu, b)
+            got = nb.GpuNewtonParallelGravity(1.0, BATCH_EPS ** 2, PRECS[k]) \
+                .calculate_accelerations(gpu, db)
+            db.close()
+            errs[k].append(rel_err_vec(got, ref))
+    out = {}
+    for k in forms:
+        e = np.concatenate(errs[k])
+        out[f"{k}_max_rel"] = float(e.max())
+        out[f"{k}_median_rel"] = float(np.median(e))
+    out["force_sample"] = (f"{len(pick)} systems (N {int(ns[pick].min())}..{int(ns[pick].max())}), "
+                           "every body, against the oracle's f64 direct sum")
+    return out
+
+
+def run_batched(args, gpu):
+    ns, systems, integ = batched_systems()
+    nsys = len(ns)
     evals = np.array([1, 1, 4])[integ]
-    inter = float(np.sum(ns * (ns - 1) * evals)) * args.steps
-    forms = f32_forms(args) + ["f64"]
-    precs = dict(F32_FORMS, f64=nb.Precision.F64)
-    rate = {}
+    inter_step = float(np.sum(ns * (ns - 1) * evals))
+    steps = args.batched_steps
+    inter = inter_step * steps
+    head = args.f32_batched
+    forms = [head, other_f32(head), "f64"]
+    rate, ms, acc = {}, {}, {}
+    roofs = {}
     for name in forms:
-        prec = precs[name]
-        bt = nb.Batch(gpu, systems, integ, 1e-3, 1.0, 1e-6, prec)
+        bt = nb.Batch(gpu, systems, integ, 1e-3, 1.0, BATCH_EPS ** 2, PRECS[name])
         bt.step(args.warmup)
         gpu.synchronize()
+        e0 = bt.energy()
         gpu.record(0)
-        bt.step(args.steps)
+        bt.step(steps)
         gpu.record(1)
-        ms = gpu.elapsed_ms(0, 1)
-        rate[name] = inter / (ms / 1e3)
-        log(f"batched {prec.name}: {ms:.1f} ms -> {rate[name]:.4e} int/s")
+        ms[name] = gpu.elapsed_ms(0, 1)
+        rate[name] = inter / (ms[name] / 1e3)
+        e1 = bt.energy()
+        d = np.abs(e1[:, 2] - e0[:, 2]) / np.abs(e0[:, 2])
+        acc[f"{name}_energy_drift_max"] = float(d.max())
+        acc[f"{name}_energy_drift_median"] = float(np.median(d))
+        # launch-level timing: a separate pass of the same steps without the
+        # graph, every force and combine launch bracketed by events
+        kt_steps = 20
+        gpu.kernel_timing(True)
+        bt.step(kt_steps)
+        kt = gpu.kernel_times()
+        gpu.kernel_timing(False)
+        roofs[name] = roofline("f64" if name == "f64" else "f32", inter_step * kt_steps, kt,
+                               rate[name])
+        roofs[name]["basis"] += f" (a separate {kt_steps}-step pass without the graph)"
+        log(f"batched {name}: {ms[name]:.1f} ms for {steps} steps -> {rate[name]:.4e} int/s, "
+            f"energy drift max {d.max():.2e}; force launches {roofs[name]['achieved']:.1f} TFLOP/s")
         bt.close()
-    head, alt = forms[0], forms[1]
-    ms_head = inter / rate[head] * 1e3
+    acc.update(batch_force_accuracy(gpu, systems, ns, forms))
+    acc["energy_drift_over"] = f"the {steps} timed steps, every system"
+    alt = forms[1]
     return {
         "metric": "pair-interactions/s (batched, f32-compensated forces)",
-        "value": rate[head], "unit": "pair-interactions/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_head / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": rate[head], "unit": "pair-interactions/s",
+        "steps": steps, "warmup": args.warmup, "ms_per_step": ms[head] / steps,
         "dtype": DTYPE[head],
-        "data": "synthetic (seeded star systems)",
         "config": {"workload": "512 star systems, N log-uniform [64, 8192], "
-                               "euler/vv/rk4 1:1:1, dt 1e-3", "systems": nsys,
-                   "bodies": int(ns.sum()), "f32_form": head, "global_batch": nsys,
-                   "parallelism": "none (1 GPU)"},
-        "roofline": roofline(rate[head], "f32", ms_head / args.steps),
-        "cpu_baseline": None,
-        "f32_alt": {"form": alt, "value": rate[alt]},
-        "fp64_plain": {"value": rate["f64"], "f32_speedup": rate[head] / rate["f64"]},
+                               "euler/vv/rk4 1:1:1, dt 1e-3, softening 1e-3, "
+                               "graph-captured steps", "systems": nsys,
+                   "bodies": int(ns.sum()), "interactions_per_step": inter_step,
+                   "f32_form": head},
+        "roofline": roofs[head],
+        "f32_alt": {"form": alt, "value": rate[alt], "roofline": roofs[alt]},
+        "fp64_plain": {"value": rate["f64"], "ms_per_step": ms["f64"] / steps,
+                       "roofline": roofs["f64"], "f32_speedup": rate[head] / rate["f64"]},
+        "accuracy": acc,
     }
 
 
-def run_serve(args, gpu):
-    """Poisson request stream against the continuous-batching server."""
-    import threading
+# ---- config 4: served requests ------------------------------------------------
+SERVE_EPS = 1e-2
+
+
+def serve_requests(nreq, mix):
     rng = np.random.default_rng(11)
-    rates = [float(r) for r in args.rates.split(",")]
-    nreq = args.requests
-    # pre-generate the request stream (host side, not timed)
-    kinds = args.mix.split(",")
+    kinds = mix.split(",")
     make = {"plummer": nb.plummer, "star": nb.star_system, "disc": nb.disc_galaxy}
     reqs = []
     for k in range(nreq):
         n = int(np.exp(rng.uniform(np.log(32), np.log(16384))))
         kind = kinds[int(rng.random() * len(kinds))]
         b = make[kind](n, 50_000 + k)
-        reqs.append(dict(b=b, integ=nb.IntegratorMethod(int(rng.integers(0, 3))),
+        reqs.append(dict(b=b, kind=kind, integ=nb.IntegratorMethod(int(rng.integers(0, 3))),
                          dt=float(rng.choice([1e-3, 5e-4])),
                          steps=int(rng.integers(10, 1001))))
-    results = []
+    return reqs
+
+
+def serve_stream(reqs, rate, prec, keep=()):
+    """Submits the request list at Poisson times of the given rate and waits
+    for every DONE.  Returns the per-rate figures and the final states of the
+    requests listed in `keep` (by index)."""
+    srv = nb.Server(capacity_bodies=4 << 20, max_systems=4096, snapshot_every=10,
+                    precision=prec)
+    gaps = np.random.default_rng(int(rate * 1000) + 17).exponential(1.0 / rate, len(reqs))
+    lat, done = [], [0]
+    finals, ids = {}, {}
+    errors = []
+    lock = threading.Lock()
+
+    def consume():
+        while done[0] < len(reqs) and not errors:
+            r = srv.poll(timeout_ms=2000, capacity=16384)
+            if r is None:
+                continue
+            ev, data = r
+            if ev["kind"] in (nb.EVENT_DONE, nb.EVENT_FAILED, nb.EVENT_CANCELLED):
+                if ev["kind"] != nb.EVENT_DONE:
+                    errors.append(ev["message"])
+                    return
+                lat.append(ev["t_event"] - ev["t_submit"])
+                with lock:
+                    k = ids.get(ev["id"])
+                if k is not None:
+                    finals[k] = data
+                done[0] += 1
+
+    th = threading.Thread(target=consume)
+    th.start()
+    t0 = time.perf_counter()
+    t_next = t0
+    for k, q in enumerate(reqs):
+        t_next += gaps[k]
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        rid = srv.submit(q["b"], q["integ"], q["dt"], q["steps"], eps2=SERVE_EPS ** 2,
+                         want_snapshots=True)
+        if k in keep:
+            with lock:
+                ids[rid] = k
+    th.join(timeout=600)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    if errors or th.is_alive():
+        raise RuntimeError(f"served stream failed: {errors or 'timed out'}")
+    lat_ms = np.array(lat) * 1e3
+    res = {"rate_req_s": rate, "requests": len(reqs), "wall_s": wall,
+           "solves_s": len(reqs) / wall, "p50_ms": float(np.percentile(lat_ms, 50)),
+           "p99_ms": float(np.percentile(lat_ms, 99)),
+           "interactions_s": st["interactions"] / wall,
+           "shared_steps": st["shared_steps"]}
+    return res, finals
+
+
+def serve_accuracy(gpu, reqs, finals, forms):
+    """Force error of sampled requests' initial states in each form against
+    the oracle, and the relative energy drift of their served results."""
+    import oracle
+    acc = {}
+    errs = {k: [] for k in forms}
+    for k in sorted(finals):
+        b = reqs[k]["b"]
+        ref = oracle.newton_parallel(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, SERVE_EPS,
+                                     host_cpus()[0])
+        for f in forms:
+            db = nb.DeviceBodies(gpu, b)
+            got = nb.GpuNewtonParallelGravity(1.0, SERVE_EPS ** 2, PRECS[f]) \
+                .calculate_accelerations(gpu, db)
+            db.close()
+            errs[f].append(rel_err_vec(got, ref))
+    for f in forms:
+        e = np.concatenate(errs[f])
+        acc[f"{f}_max_rel"] = float(e.max())
+        acc[f"{f}_median_rel"] = float(np.median(e))
+    drifts = []
+    for k in sorted(finals):
+        b = reqs[k]["b"]
+        x, y, z, vx, vy, vz = finals[k]
+        e = []
+        for bb in (b, nb.Bodies(b.masses, x, y, z, vx, vy, vz)):
+            db = nb.DeviceBodies(gpu, bb)
+            e.append(energy(gpu, db, 1.0, SERVE_EPS))
+            db.close()
+        drifts.append(abs(e[1][2] - e[0][2]) / abs(e[0][2]))
+    acc["energy_drift_max"] = float(np.max(drifts))
+    acc["energy_drift_median"] = float(np.median(drifts))
+    acc["sample"] = (f"{len(finals)} requests (N {min(len(reqs[k]['b']) for k in finals)}.."
+                     f"{max(len(reqs[k]['b']) for k in finals)}), every body; energy drift "
+                     "of the served result after the request's own steps")
+    return acc
+
+
+def run_serve(args, gpu):
+    reqs = serve_requests(args.requests, args.mix)
+    rates = [float(r) for r in args.rates.split(",")]
+    head = args.f32_serve
+    keep = set(range(0, len(reqs), max(1, len(reqs) // 12)))
+    results, finals = [], {}
     for rate in rates:
-        srv = nb.Server(capacity_bodies=4 << 20, max_systems=4096, snapshot_every=10,
-                        precision=F32_FORMS[args.f32])
-        gaps = rng.exponential(1.0 / rate, nreq)
-        lat, done, interactions = [], [0], 0.0
-        t_first = [None]
-
-        def consume():
-            while done[0] < nreq:
-                r = srv.poll(timeout_ms=2000, capacity=16384)
-                if r is None:
-                    continue
-                ev, _ = r
-                if ev["kind"] in (nb.EVENT_DONE, nb.EVENT_FAILED, nb.EVENT_CANCELLED):
-                    if ev["kind"] != nb.EVENT_DONE:
-                        raise RuntimeError(ev["message"])
-                    lat.append(ev["t_event"] - ev["t_submit"])
-                    done[0] += 1
-
-        th = threading.Thread(target=consume)
-        th.start()
-        t0 = time.perf_counter()
-        t_next = t0
-        for k, q in enumerate(reqs):
-            t_next += gaps[k]
-            d = t_next - time.perf_counter()
-            if d > 0:
-                time.sleep(d)
-            srv.submit(q["b"], q["integ"], q["dt"], q["steps"], eps2=1e-4,
-                       want_snapshots=True)
-        th.join(timeout=600)
-        wall = time.perf_counter() - t0
-        st = srv.stats()
-        srv.close()
-        lat_ms = np.array(lat) * 1e3
-        res = {"rate_req_s": rate, "requests": nreq, "wall_s": wall,
-               "solves_s": nreq / wall, "p50_ms": float(np.percentile(lat_ms, 50)),
-               "p99_ms": float(np.percentile(lat_ms, 99)),
-               "interactions_s": st["interactions"] / wall,
-               "shared_steps": st["shared_steps"]}
-        log(f"serve rate {rate}/s: {res}")
+        res, fin = serve_stream(reqs, rate, PRECS[head], keep if not finals else ())
+        finals = finals or fin
+        log(f"serve {head} rate {rate}/s: {res['solves_s']:.1f} solves/s, "
+            f"p50 {res['p50_ms']:.0f} ms, p99 {res['p99_ms']:.0f} ms, "
+            f"{res['interactions_s']:.3e} int/s")
         results.append(res)
-    best = max(results, key=lambda r: r["interactions_s"])
+    sat = max(rates)
+    res64, _ = serve_stream(reqs, sat, PRECS["f64"])
+    log(f"serve f64 rate {sat}/s: {res64['solves_s']:.1f} solves/s, "
+        f"{res64['interactions_s']:.3e} int/s")
+    top = max(results, key=lambda r: r["rate_req_s"])
+    acc = serve_accuracy(gpu, reqs, finals, [head, "f64"])
     return {
-        "metric": "served solves/s (p50/p99 latency per rate in `serving`)",
-        "value": best["solves_s"], "unit": "solves/s", "n_gpus": 1,
-        "steps": None, "warmup": 0, "ms_per_step": None,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": DTYPE[args.f32],
-        "data": f"synthetic Poisson request stream ({args.mix.replace(',', ' / ')})",
-        "config": {"workload": "served requests, N log-uniform [32, 16384], steps "
-                               "uniform [10, 1000], random integrator, dt in "
-                               "{1e-3, 5e-4}, snapshots every 10 steps",
-                   "requests_per_rate": nreq, "rates": rates,
-                   "parallelism": "none (1 GPU)"},
-        "roofline": None, "cpu_baseline": None, "serving": results,
+        "metric": "served solves/s at saturation (p50/p99 latency per rate in `rates`)",
+        "value": top["solves_s"], "unit": "solves/s", "dtype": DTYPE[head],
+        "config": {"workload": "served requests, Poisson arrivals, N log-uniform "
+                               "[32, 16384], steps uniform [10, 1000], random "
+                               "integrator, dt in {1e-3, 5e-4}, softening 1e-2, "
+                               "snapshots every 10 steps",
+                   "requests_per_rate": len(reqs), "mix": args.mix, "f32_form": head},
+        "rates": results,
+        "fp64_plain": {"rate_req_s": sat, "solves_s": res64["solves_s"],
+                       "p50_ms": res64["p50_ms"], "p99_ms": res64["p99_ms"],
+                       "interactions_s": res64["interactions_s"],
+                       "f32_speedup": top["interactions_s"] / res64["interactions_s"]},
+        "roofline": {"bound": "valu", "achieved": top["interactions_s"] * FLOPS_PER_INTERACTION / 1e12,
+                     "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
+                     "frac": top["interactions_s"] * FLOPS_PER_INTERACTION / 1e12 / PEAK_TFLOPS["f32"],
+                     "traffic": None,
+                     "basis": "interactions evaluated over the wall time of the saturated "
+                              "stream (submit of the first request to DONE of the last)"},
+        "accuracy": acc,
     }
 
 
+# ---- config 5: N = 1,048,576 colliding spheres, RK4 ---------------------------
 def run_large(args, gpu):
-    """Two colliding Plummer spheres, N=1,048,576, RK4 (4 evaluations/step)."""
     n, g, eps, dt = 1 << 20, 1.0, 0.002, 2.5e-4
     b = nb.colliding(n, 5, separation=10.0, rel_speed=0.5)
-    forms = f32_forms(args) + ["f64"]
-    precs = dict(F32_FORMS, f64=nb.Precision.F64)
-    # sampled-target oracle: 4096 random targets against all sources (f64 CPU)
+    head = args.f32 or "f32r"
+    forms = [head, other_f32(head), "f64"]
+    steps = {head: args.large_steps, forms[1]: max(1, args.large_steps // 4),
+             "f64": args.large_steps}
     acc_rep = {}
     if not args.no_cpu_baseline:
         import oracle
         idx = np.sort(np.random.default_rng(9).choice(n, 4096, replace=False))
         t0 = time.perf_counter()
         ref = oracle.newton_targets(b.masses, b.pos_x, b.pos_y, b.pos_z, g, eps, idx,
-                                    min(16, oracle.default_threads()))
-        log(f"sampled oracle: 4096 targets in {time.perf_counter() - t0:.1f} s")
+                                    host_cpus()[0])
+        log(f"large: sampled oracle, 4096 targets in {time.perf_counter() - t0:.1f} s")
         for name in forms:
             db = nb.DeviceBodies(gpu, b)
-            got = nb.GpuNewtonParallelGravity(g, eps * eps, precs[name]) \
+            got = nb.GpuNewtonParallelGravity(g, eps * eps, PRECS[name]) \
                 .calculate_accelerations(gpu, db)
             db.close()
             mx, med = rel_err(tuple(a[idx] for a in got), ref)
             acc_rep[f"{name}_max_rel_sampled"] = mx
             acc_rep[f"{name}_median_rel_sampled"] = med
-    inter = n * (n - 1) * 4 * args.steps
-    rate, ms = {}, {}
+        acc_rep["sample"] = "4096 seeded targets against all sources (oracle, f64)"
+    inter_eval = n * (n - 1)
+    rate, ms, roofs = {}, {}, {}
     for name in forms:
-        grav = nb.GpuNewtonParallelGravity(g, eps * eps, precs[name])
+        grav = nb.GpuNewtonParallelGravity(g, eps * eps, PRECS[name])
         integ = nb.GpuRungeKuttaIntegrator(dt)
         db = nb.DeviceBodies(gpu, b)
         e0 = energy(gpu, db, g, eps)
         gpu.synchronize()
+        gpu.kernel_timing(True)
         gpu.record(0)
-        integ.step(gpu, db, grav, args.steps)
+        integ.step(gpu, db, grav, steps[name])
         gpu.record(1)
         ms[name] = gpu.elapsed_ms(0, 1)
+        kt = gpu.kernel_times()
+        gpu.kernel_timing(False)
         e1 = energy(gpu, db, g, eps)
+        inter = inter_eval * 4 * steps[name]
         rate[name] = inter / (ms[name] / 1e3)
+        roofs[name] = roofline("f64" if name == "f64" else "f32",
+                               inter_eval * kt["force_launches"], kt, rate[name])
         acc_rep[f"{name}_energy_drift"] = abs(e1[2] - e0[2]) / abs(e0[2])
-        log(f"large {name}: {ms[name]:.0f} ms for {args.steps} RK4 steps, "
-            f"energy drift {acc_rep[f'{name}_energy_drift']:.3e}")
+        acc_rep[f"{name}_energy_drift_steps"] = steps[name]
+        log(f"large {name}: {ms[name]:.0f} ms for {steps[name]} RK4 steps -> "
+            f"{rate[name]:.4e} int/s, energy drift {acc_rep[f'{name}_energy_drift']:.3e}")
         db.close()
-    head, alt = forms[0], forms[1]
+    alt = forms[1]
     return {
         "metric": "pair-interactions/s (f32-compensated forces, f64 state)",
-        "value": rate[head], "unit": "pair-interactions/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": 0, "ms_per_step": ms[head] / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": rate[head], "unit": "pair-interactions/s",
+        "steps": steps[head], "warmup": 0, "ms_per_step": ms[head] / steps[head],
         "dtype": DTYPE[head],
-        "data": "synthetic (two seeded Plummer spheres)",
         "config": {"workload": "two colliding Plummer spheres N=1048576, softening "
                                "0.002, dt 2.5e-4, runge-kutta", "n": n,
-                   "f32_form": head, "parallelism": "none (1 GPU)"},
-        "roofline": roofline(rate[head], "f32", ms[head] / args.steps / 4),
-        "cpu_baseline": None,
-        "f32_alt": {"form": alt, "value": rate[alt],
-                    "roofline": roofline(rate[alt], "f32", ms[alt] / args.steps / 4)},
-        "fp64_plain": {"value": rate["f64"], "f32_speedup": rate[head] / rate["f64"],
-                       "roofline": roofline(rate["f64"], "f64", ms["f64"] / args.steps / 4)},
+                   "f32_form": head},
+        "roofline": roofs[head],
+        "f32_alt": {"form": alt, "value": rate[alt], "steps": steps[alt],
+                    "roofline": roofs[alt]},
+        "fp64_plain": {"value": rate["f64"], "ms_per_step": ms["f64"] / steps["f64"],
+                       "roofline": roofs["f64"], "f32_speedup": rate[head] / rate["f64"]},
         "accuracy": acc_rep,
     }
 
 
+# ---- Barnes-Hut (not in the default run) ------------------------------------
 def run_bh(args, gpu):
     """Barnes-Hut, theta 0.5, on the N=1,048,576 colliding spheres: the tree
     is rebuilt for every evaluation, as the reference does.  Timed: fast-f64
@@ -471,7 +659,7 @@ def run_bh(args, gpu):
     step_ms = gpu.elapsed_ms(0, 1) / args.steps
     e1 = energy(gpu, db, g, eps)
     db.close()
-    acc = {"fast_vs_direct_max_rel": None, "fast_vs_direct_median_rel": None}
+    acc = {}
     mx, med = rel_err(acc_fast, acc_direct)
     acc.update(fast_vs_direct_max_rel=mx, fast_vs_direct_median_rel=med,
                strict_vs_direct_median_rel=rel_err(acc_strict, acc_direct)[1],
@@ -480,7 +668,7 @@ def run_bh(args, gpu):
     if not args.no_cpu_baseline:
         import oracle
         idx = np.sort(np.random.default_rng(9).choice(n, 4096, replace=False))
-        threads = min(16, oracle.default_threads())
+        threads = host_cpus()[0]
         t0 = time.perf_counter()
         ref = oracle.barnes_hut_targets(b.masses, b.pos_x, b.pos_y, b.pos_z, g, eps,
                                         theta, idx, threads)
@@ -551,24 +739,40 @@ def run_copy(args, gpu):
             "roofline": None, "cpu_baseline": None}
 
 
+SUB_COMMON = {"n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+              "vs_baseline": None, "parallelism": "none (1 GPU)"}
+
+
+def with_common(rec, data):
+    out = dict(SUB_COMMON)
+    out["data"] = data
+    out.update(rec)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps of the headline (single) workload")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload",
-                    choices=["single", "batched", "serve", "large", "bh", "copy"],
-                    default="single")
-    ap.add_argument("--rates", default="20,50,80,160,640",
-                    help="serve: request rates (1/s); the last ones saturate the GPU")
+                    choices=["all", "single", "batched", "serve", "large", "bh", "copy"],
+                    default="all",
+                    help="all (default): single as the headline with batched, "
+                         "serving and large as sub-records of the same line")
+    ap.add_argument("--batched-steps", type=int, default=500)
+    ap.add_argument("--large-steps", type=int, default=20)
+    ap.add_argument("--rates", default="20,80,640",
+                    help="serve: request rates (1/s); the last one saturates the GPU")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests per rate")
     ap.add_argument("--mix", default="plummer,star",
                     help="serve: request scenarios, from plummer, star, disc")
     ap.add_argument("--f32", choices=sorted(F32_FORMS), default=None,
-                    help="the f32 form of the headline value (the other one is "
-                         "timed beside it); default: the faster one for the "
-                         "workload (f32r for one large system, f32c for many "
-                         "small ones)")
+                    help="f32 form of the single and large headlines (default f32r; "
+                         "the other form is timed beside it)")
+    ap.add_argument("--f32-batched", choices=sorted(F32_FORMS), default="f32r")
+    ap.add_argument("--f32-serve", choices=sorted(F32_FORMS), default="f32r")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
@@ -578,17 +782,42 @@ def main():
               "system would shard by target range with one all-gather of source "
               "positions per force evaluation -- not built here)", file=sys.stderr)
         sys.exit(2)
-    if args.f32 is None:
-        args.f32 = default_f32(args.workload)
     if args.steps is None:
-        args.steps = {"single": 200, "batched": 50, "serve": 0, "large": 2,
+        args.steps = {"single": 200, "all": 200, "batched": 50, "serve": 0, "large": 2,
                       "bh": 5, "copy": 1}[args.workload]
     gpu = nb.GpuContext(0)
     name, cus = gpu.device_info()
     log(f"device {name}, {cus} CUs")
-    rec = {"single": run_single, "batched": run_batched, "serve": run_serve,
-           "large": run_large, "bh": run_bh, "copy": run_copy}[args.workload](args, gpu)
+    t_start = time.perf_counter()
+    if args.workload == "all":
+        rec = run_single(args, gpu)
+        rec["batched"] = with_common(run_batched(args, gpu),
+                                     "synthetic (seeded star systems)")
+        rec["serving"] = with_common(run_serve(args, gpu),
+                                     f"synthetic Poisson request stream "
+                                     f"({args.mix.replace(',', ' / ')})")
+        rec["large"] = with_common(run_large(args, gpu),
+                                   "synthetic (two seeded Plummer spheres)")
+    elif args.workload == "single":
+        rec = run_single(args, gpu)
+    else:
+        if args.workload == "batched":
+            args.batched_steps = args.steps
+        if args.workload == "large":
+            args.large_steps = args.steps
+        fn = {"batched": run_batched, "serve": run_serve, "large": run_large,
+              "bh": run_bh, "copy": run_copy}[args.workload]
+        rec = fn(args, gpu)
+        for k, v in SUB_COMMON.items():
+            if k != "parallelism":
+                rec.setdefault(k, v)
+        rec.setdefault("steps", args.steps)
+        rec.setdefault("warmup", args.warmup)
+        rec.setdefault("cpu_baseline", None)
+        rec.setdefault("data", "synthetic")
+        rec.setdefault("higher_is_better", True)
     rec["device"] = {"arch": name, "compute_units": cus}
+    rec["bench_wall_s"] = time.perf_counter() - t_start
     print(json.dumps(rec), flush=True)
 
 

@@ -90,6 +90,16 @@ int nbody_ctx_device_info(nbody_ctx *ctx, char *name, int name_len,
  * recorded slots (synchronises on the later one). */
 int nbody_ctx_record(nbody_ctx *ctx, int slot);
 int nbody_ctx_elapsed(nbody_ctx *ctx, int slot_from, int slot_to, float *ms);
+/* Launch timing (no reference counterpart; the reference times whole steps
+ * under criterion, benches/gpu_integrator_bench.rs:155-162): with enable != 0
+ * every force and combine launch of this context's bodies and batches is
+ * bracketed by two HIP events on the context's stream (batched stepping then
+ * launches without its graph).  Enabling or disabling zeroes the sums.
+ * nbody_ctx_kernel_times waits for the recorded launches and returns out[0]
+ * force-launch milliseconds, out[1] force launches, out[2] combine-launch
+ * milliseconds, out[3] combine launches, summed since timing was enabled. */
+int nbody_ctx_kernel_timing(nbody_ctx *ctx, int enable);
+int nbody_ctx_kernel_times(nbody_ctx *ctx, double out[4]);
 
 /* ---- tuning (no reference counterpart): source-range parts per target
  * tile for systems of n bodies, by bucket b = floor(log2 n), b = 0..31.

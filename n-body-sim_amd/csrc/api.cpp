@@ -52,6 +52,7 @@ struct nbody_ctx {
   std::string name;
   int cus = 0;
   hipEvent_t ev[8] = {};
+  nbody::KernelTimer timer;  // force/combine launch timing of its children
   // Bodies and batches made from this context share its stream.  Destroying
   // the context while some live only marks it closed; the last of them to be
   // destroyed frees it, so destroy order does not matter to callers.
@@ -260,6 +261,27 @@ int nbody_ctx_elapsed(nbody_ctx* ctx, int a, int b, float* ms) {
   });
 }
 
+int nbody_ctx_kernel_timing(nbody_ctx* ctx, int enable) {
+  return guarded([&] {
+    require(ctx != nullptr, NBODY_E_INVALID, "null context");
+    ctx->use();
+    ctx->timer.reset();
+    ctx->timer.on = enable != 0;
+  });
+}
+
+int nbody_ctx_kernel_times(nbody_ctx* ctx, double out[4]) {
+  return guarded([&] {
+    require(ctx && out, NBODY_E_INVALID, "null argument");
+    ctx->use();
+    ctx->timer.harvest();
+    out[0] = ctx->timer.ms[nbody::KernelTimer::FORCE];
+    out[1] = (double)ctx->timer.launches[nbody::KernelTimer::FORCE];
+    out[2] = ctx->timer.ms[nbody::KernelTimer::COMBINE];
+    out[3] = (double)ctx->timer.launches[nbody::KernelTimer::COMBINE];
+  });
+}
+
 // ---- tuning ---------------------------------------------------------------
 int nbody_tuning_set_split(const int* parts, int count) {
   return guarded([&] {
@@ -291,6 +313,7 @@ int nbody_bodies_create(nbody_ctx* ctx, int64_t n, const double* masses,
     b->n = n;
     b->eng = std::make_unique<Engine>(ctx->device, ctx->stream,
                                       nbody::MODE_F64, n, 1);
+    b->eng->set_timer(&ctx->timer);
     nbody::SystemParams p;
     p.n = n;
     b->id = b->eng->add(p, masses, pos_x, pos_y, pos_z, vel_x, vel_y, vel_z);
@@ -468,6 +491,7 @@ int nbody_batch_create(nbody_ctx* ctx, int nsys, const int64_t* n,
     // hundreds of resident systems fill the GPU with waves already: the
     // batched form keeps each system to at most 8 source parts
     bt->eng->set_split_cap(8);
+    bt->eng->set_timer(&ctx->timer);
     for (int s = 0; s < nsys; ++s) {
       nbody::SystemParams p;
       p.n = n[s];

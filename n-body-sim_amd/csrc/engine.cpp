@@ -21,6 +21,46 @@ void hip_check(hipError_t e, const char* what) {
   throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+KernelTimer::~KernelTimer() {
+  for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
+}
+
+void KernelTimer::begin(hipStream_t s) {
+  if (used_ + 2 > ev_.size()) {
+    if (ev_.size() >= 4096) harvest();  // pool full: wait, then reuse
+    while (used_ + 2 > ev_.size()) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      ev_.push_back(e);
+    }
+  }
+  hip_check(hipEventRecord(ev_[used_++], s), "hipEventRecord");
+}
+
+void KernelTimer::end(hipStream_t s, Kind kind) {
+  hip_check(hipEventRecord(ev_[used_++], s), "hipEventRecord");
+  kind_.push_back(kind);
+}
+
+void KernelTimer::harvest() {
+  for (size_t k = 0; k < kind_.size(); ++k) {
+    hip_check(hipEventSynchronize(ev_[2 * k + 1]), "hipEventSynchronize");
+    float t = 0.f;
+    hip_check(hipEventElapsedTime(&t, ev_[2 * k], ev_[2 * k + 1]),
+              "hipEventElapsedTime");
+    ms[kind_[k]] += t;
+    ++launches[kind_[k]];
+  }
+  kind_.clear();
+  used_ = 0;
+}
+
+void KernelTimer::reset() {
+  harvest();
+  ms[0] = ms[1] = 0.0;
+  launches[0] = launches[1] = 0;
+}
+
 namespace {
 constexpr int64_t kAlign = 64;  // system ranges start on a wave boundary
 
@@ -125,6 +165,7 @@ size_t Engine::carve(char* base) {
     A_.hz[b] = D();
   }
   A_.perm = (int32_t*)F();
+  A_.iperm = (int32_t*)F();
   A_.gm32 = F();
   A_.hg = F();
   A_.cap = cap_;
@@ -273,9 +314,11 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
     s.com[q] = (M != 0.0) ? c[q] / M : 0.0;
     s.vcom[q] = (M != 0.0) ? v[q] / M : 0.0;
   }
-  for (int64_t i = 0; i < p.n; ++i)
+  for (int64_t i = 0; i < p.n; ++i) {
     s.ext = std::max({s.ext, std::fabs(x[i] - s.com[0]),
                       std::fabs(y[i] - s.com[1]), std::fabs(z[i] - s.com[2])});
+    s.mmax = std::max(s.mmax, std::fabs(m[i]));
+  }
   const size_t bytes = sizeof(double) * p.n;
   double* dst[7] = {A_.m, A_.x, A_.y, A_.z, A_.vx, A_.vy, A_.vz};
   for (int q = 0; q < 7; ++q)
@@ -411,7 +454,15 @@ SysDesc Engine::desc_of(int id) const {
   d.eps2 = s.p.eps2;
   d.g = s.p.g;
   d.eps2f = (float)s.p.eps2;
-  d.nsplit = split_of(id);
+  d.nsplit = (int16_t)split_of(id);
+  d.flags = 0;
+  if (d.eps2f > 0.0f) {
+    // the self term's k = g*m * rsq(eps2f)^3 in f32: finite with a margin
+    // for the rounding of g*m and the approximate v_rsq_f32 (a few ulp)
+    const double inv = 1.0 / std::sqrt((double)d.eps2f);
+    const double kmax = std::fabs(s.p.g) * s.mmax * ((inv * inv) * inv);
+    if (kmax < 0x1p120) d.flags |= SYS_F32_SELF_FINITE;
+  }
   return d;
 }
 
@@ -489,12 +540,8 @@ void Engine::run_round(std::vector<WorkItem>& items,
                            hipMemcpyHostToDevice, stream_),
             "hipMemcpy(work list)");
   const int nf = (int)items.size(), nc = (int)combine.size();
-  hipError_t e = potential
-                     ? launch_potential(d_arena_, d_items_, nf, d_sys_, stream_)
-                     : launch_force(mode_, d_arena_, d_items_, nf, d_sys_, stream_);
-  hip_check(e, "force launch");
-  hip_check(launch_combine(mode_, d_arena_, d_items_ + nf, nc, d_sys_, stream_),
-            "combine launch");
+  force_launch(d_items_, nf, potential);
+  combine_launch(d_items_ + nf, nc);
   // the pageable host list must not change before the copy is consumed
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
@@ -703,16 +750,31 @@ void Engine::upload_step_lists(const std::vector<int>& ids, StepLists& L) {
   step_lists_ = L;
 }
 
+void Engine::force_launch(const WorkItem* items, int nitems, bool potential) {
+  if (nitems <= 0) return;
+  const bool timed = timer_ && timer_->on;
+  if (timed) timer_->begin(stream_);
+  hip_check(potential ? launch_potential(d_arena_, items, nitems, d_sys_, stream_)
+                      : launch_force(mode_, d_arena_, items, nitems, d_sys_, stream_),
+            "force launch");
+  if (timed) timer_->end(stream_, KernelTimer::FORCE);
+}
+
+void Engine::combine_launch(const WorkItem* items, int nitems) {
+  if (nitems <= 0) return;
+  const bool timed = timer_ && timer_->on;
+  if (timed) timer_->begin(stream_);
+  hip_check(launch_combine(mode_, d_arena_, items, nitems, d_sys_, stream_),
+            "combine launch");
+  if (timed) timer_->end(stream_, KernelTimer::COMBINE);
+}
+
 void Engine::launch_step(const StepLists& L, int which) {
   for (int r = 0; r < 4; ++r) {
     const int k = which * 8 + r * 2;
     if (L.cnt[k] == 0) continue;
-    hip_check(launch_force(mode_, d_arena_, d_step_items_ + L.off[k], L.cnt[k],
-                           d_sys_, stream_),
-              "force launch");
-    hip_check(launch_combine(mode_, d_arena_, d_step_items_ + L.off[k + 1],
-                             L.cnt[k + 1], d_sys_, stream_),
-              "combine launch");
+    force_launch(d_step_items_ + L.off[k], L.cnt[k]);
+    combine_launch(d_step_items_ + L.off[k + 1], L.cnt[k + 1]);
   }
 }
 
@@ -734,6 +796,7 @@ void Engine::run_steps(const std::vector<int>& ids, int64_t nsteps,
   if (nsteps < 0) throw Error(NBODY_E_INVALID, "nsteps < 0");
   if (graph && gravity_ != GRAV_DIRECT)
     throw Error(NBODY_E_UNSUPPORTED, "graph stepping runs the direct sum only");
+  if (timer_ && timer_->on) graph = false;  // launches bracketed one by one
   for (int id : ids) check_id(id);
   if (nsteps == 0 || ids.empty()) return;
   prepare(ids);

@@ -32,6 +32,31 @@ struct Error : std::runtime_error {
 
 void hip_check(hipError_t e, const char* what);
 
+// Optional timing of force and combine launches: while `on`, each launch is
+// bracketed by two HIP events on the engine's stream (from a pool, reused
+// after every harvest) and harvest() adds up the elapsed times per kind.
+// One timer serves every engine of a context; use from one thread.
+class KernelTimer {
+ public:
+  enum Kind { FORCE = 0, COMBINE = 1 };
+  KernelTimer() = default;
+  KernelTimer(const KernelTimer&) = delete;
+  KernelTimer& operator=(const KernelTimer&) = delete;
+  ~KernelTimer();
+  bool on = false;
+  void begin(hipStream_t s);          // before a launch
+  void end(hipStream_t s, Kind kind);  // after it
+  void harvest();                     // waits for the recorded launches
+  void reset();                       // harvest, then zero the sums
+  double ms[2] = {0.0, 0.0};
+  int64_t launches[2] = {0, 0};
+
+ private:
+  std::vector<hipEvent_t> ev_;
+  std::vector<Kind> kind_;
+  size_t used_ = 0;  // events recorded since the last harvest
+};
+
 struct SystemParams {
   int64_t n = 0;
   int integ = INTEG_EULER;
@@ -62,6 +87,9 @@ class Engine {
   void set_split_cap(int cap) { split_cap_ = cap < 1 ? 1 : cap; }
   // Octree nodes of a system's last Barnes-Hut build (synchronises).
   int64_t tree_nodes(int id);
+  // Launch timing (may be null).  While it is on, graph stepping launches
+  // the same kernels directly so that each one can be bracketed.
+  void set_timer(KernelTimer* t) { timer_ = t; }
 
   // Uploads a system; returns its id.  Throws Error(NBODY_E_NOSPACE) when no
   // contiguous range of the arena is free.
@@ -165,6 +193,7 @@ class Engine {
     double com[3] = {0, 0, 0};  // centre of mass at upload (sort box centre,
     double vcom[3] = {0, 0, 0};  // advanced with the centre-of-mass velocity)
     double ext = 0.0;           // largest coordinate distance from com
+    double mmax = 0.0;          // largest |m| (bounds the f32 self term)
     int64_t sorted_at = -1;     // step count of the last body ordering
   };
   // MODE_F32R body ordering.  A system's bodies are put in Hilbert order
@@ -220,7 +249,12 @@ class Engine {
   // One step of every listed system with Barnes-Hut forces (parity t & 1).
   void bh_step(const std::vector<int>& ids, int64_t t);
 
+  // launch_force / launch_combine of nitems items, bracketed by the timer
+  void force_launch(const WorkItem* items, int nitems, bool potential = false);
+  void combine_launch(const WorkItem* items, int nitems);
+
   hipStream_t stream_;
+  KernelTimer* timer_ = nullptr;
   int mode_;
   int gravity_ = GRAV_DIRECT;
   int split_cap_ = kMaxSplit;

@@ -21,6 +21,7 @@
 // target of a running launch ever reads a value that launch writes.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <hipcub/hipcub.hpp>
 #include <type_traits>
 
@@ -380,18 +381,33 @@ __device__ __forceinline__ bool near_source(const TargetR& t, float sx,
   return !(a * a <= (kNearK * kNearK) * d2);
 }
 
+// The value in a VGPR.  Uniform values otherwise stay in SGPRs, and on
+// gfx950 an f32 VALU instruction with an SGPR operand issues at about 4.2
+// cycles per wave64 against about 2.4 with VGPR operands only (measured).
+__device__ __forceinline__ float in_vgpr(float v) {
+  // a DPP move with the identity lane pattern (quad_perm [0,1,2,3])
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xe4, 0xf,
+                                      0xf, false));
+}
+
 // One staged chunk of 64 sources against the lane's target.  RES adds the
 // residual correction (near chunks).  Group sums of 32 sources are added with
-// Kahan compensation.  NOMASK: eps2 > 0, so a source at the target's own
-// position (itself) contributes exactly zero; otherwise pairs with r2 == 0
-// are masked (coincident bodies then add nothing, where the reference's f64
-// path would produce NaN).  Sources past the system's end stage as zero mass.
+// Kahan compensation.  NOMASK (SYS_F32_SELF_FINITE): eps2 > 0 and the self
+// term's k is finite, so a source at the target's own position (itself)
+// contributes exactly zero.  Otherwise (zero softening, or g*m*eps^-3 beyond
+// the f32 range) k is clamped to +-FLT_MAX (fminf/fmaxf return the number
+// when k is NaN from 0 * inf), so a pair with a zero difference adds exactly
+// zero: the target's own term, and coincident bodies, which the reference's
+// f64 path gives NaN at zero softening and zero when softened.  Sources past
+// the system's end stage as zero mass.
 template <bool NOMASK, bool RES>
 __device__ __forceinline__ void chunk_f32r(const float4* stage,
                                            const float4* res,
-                                           const TargetR& t, float eps2,
+                                           const TargetR& t, float eps2_s,
                                            Kahan3& acc) {
   constexpr int B = RES ? kLdsBatch / 2 : kLdsBatch;
+  const float eps2 = in_vgpr(eps2_s);
 #pragma unroll 1
   for (int q = 0; q < kWave; q += 32) {
     float px = 0.f, py = 0.f, pz = 0.f;
@@ -414,7 +430,9 @@ __device__ __forceinline__ void chunk_f32r(const float4* stage,
         const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
         const float inv = __builtin_amdgcn_rsqf(r2);
         float k = s[u].w * ((inv * inv) * inv);
-        if (!NOMASK) k = r2 > 0.f ? k : 0.f;
+        // k clamped to the finite range: a zero difference then adds
+        // exactly zero (an overflowing or 0 * inf k times 0)
+        if (!NOMASK) k = fmaxf(fminf(k, FLT_MAX), -FLT_MAX);
         px = fmaf(k, dx, px);
         py = fmaf(k, dy, py);
         pz = fmaf(k, dz, pz);
@@ -520,7 +538,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const Arena* Ap,
     int64_t jlo, jhi;
     part_range<kWave>(off, n, nsplit, part, jlo, jhi);
     const float e2 = sd->eps2f;
-    if (e2 > 0.f)
+    if (sd->flags & SYS_F32_SELF_FINITE)
       sweep_f32r<true>(stage[wave][0], stage[wave][1], A->hx[src], A->hy[src],
                        A->hz[src], A->hg, jlo, jhi, end, ox, oy, oz, t, e2,
                        acc);
@@ -540,7 +558,8 @@ __global__ __launch_bounds__(kBlock) void k_force(const Arena* Ap,
     float px = 0.f, py = 0.f, pz = 0.f;
     int64_t jlo, jhi;
     part_range<U>(off, n, nsplit, part, jlo, jhi);
-    ChunkF32<SrcX64, double, U> ch{i, end, xi, yi, zi, sd->eps2f, px, py, pz};
+    ChunkF32<SrcX64, double, U> ch{i,  end, xi, yi, zi, in_vgpr(sd->eps2f),
+                                   px, py, pz};
     drive<U, KC>(src_bundle, jlo, jhi, t0, end, ch, [&] {
       kahan_add(acc.sx, acc.cx, px);
       kahan_add(acc.sy, acc.cy, py);
@@ -577,9 +596,9 @@ __global__ __launch_bounds__(kBlock) void k_force(const Arena* Ap,
   }
   if (!active) return;
   if (sd->nsplit > 1) {  // partial sum; k_combine finishes the target
-    // MODE_F32R keeps partials in slot order (k_combine walks slots)
-    const int64_t q =
-        (int64_t)it.part * A->pstride + (MODE == MODE_F32R && !POT ? slot : i);
+    // partials in body order in every mode (in MODE_F32R a wave's 64 writes
+    // are scattered, so k_combine can walk bodies with coalesced accesses)
+    const int64_t q = (int64_t)it.part * A->pstride + i;
     A->px[q] = ax;
     if (!POT) {
       A->py[q] = ay;
@@ -608,8 +627,8 @@ __global__ __launch_bounds__(kBlock) void k_combine(const Arena* Ap,
   const CArena* A = (const CArena*)Ap;
   const CSys* sd = (const CSys*)(sysd + it.sys);
   const int64_t end = sd->off + sd->n;
-  // the tile's targets: bodies, or in MODE_F32R slots in Hilbert order (the
-  // potential pass, computed in f64, is always in body order)
+  // the tile's target bodies (in body order in every mode; in MODE_F32R the
+  // force items' tiles are slot ranges, but the partials are in body order)
   const int64_t k = sd->off + (int64_t)it.tile * kWave +
                     (int64_t)(threadIdx.x & (kWave - 1));
   if (k >= end) return;
@@ -626,8 +645,9 @@ __global__ __launch_bounds__(kBlock) void k_combine(const Arena* Ap,
     ay = ay + A->py[p * st + k];
     az = az + A->pz[p * st + k];
   }
-  const int64_t i = MODE == MODE_F32R ? sd->off + A->perm[k] : k;
-  epilogue<MODE>(A, sd, it.stage, it.src ^ 1, i, k, ax, ay, az);
+  // the body's slot in Hilbert order, for MODE_F32R's slot-ordered snapshot
+  const int64_t slot = MODE == MODE_F32R ? sd->off + A->iperm[k] : k;
+  epilogue<MODE>(A, sd, it.stage, it.src ^ 1, k, slot, ax, ay, az);
 }
 
 // Builds source snapshot it.src of each body from the master state:
@@ -870,6 +890,7 @@ __global__ __launch_bounds__(kWave) void k_sort_scatter(const Arena* Ap,
   if (vals) {
     v = vals[t.base + k];
     A->perm[q] = v;
+    A->iperm[t.off + v] = k;
   } else {
     v = A->perm[q];
   }

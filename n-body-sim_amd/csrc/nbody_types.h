@@ -4,7 +4,8 @@
 //   m, gm64 (= G*m), gm32         bodies' masses, G folded in once per system
 //   x, y, z, vx, vy, vz           master state, binary64
 //   sx/sy/sz[2]                   f64 source snapshot, double-buffered
-//   perm                          MODE_F32R body order (Hilbert curve)
+//   perm, iperm                   MODE_F32R body order (Hilbert curve)
+//                                 and its inverse
 //   hx/hy/hz[2], hg               MODE_F32R: the snapshot and g*m in that
 //                                 order, so sources read coalesced
 //   ax, ay, az                    carried acceleration (velocity Verlet) or the
@@ -12,6 +13,7 @@
 //   rk[9]                         RK4 running sums and next-stage velocity
 //   phi                           per-body potential (energy diagnostics)
 //   px/py/pz[kMaxSplit]           partial sums of split source ranges
+//                                 (body order in every mode)
 // Systems occupy disjoint [off, off+n) ranges of every array.  Every array has
 // kPad elements of zeroed slack past the arena end so a source chunk that
 // straddles the last body stays in bounds.
@@ -71,9 +73,19 @@ struct SysDesc {       // one per resident system (48 B)
   double eps2;         // softening squared, f64
   double g;            // gravitational constant
   float eps2f;         // eps2 rounded to f32
-  int32_t nsplit;      // source-range parts per tile (1: no split)
+  int16_t nsplit;      // source-range parts per tile (1: no split)
+  int16_t flags;       // SysFlag bits
 };
 static_assert(sizeof(SysDesc) == 48, "SysDesc layout");
+
+enum SysFlag : int16_t {
+  // eps2f > 0 and the f32 self term g*m_i * rsq(eps2f)^3 is finite for every
+  // body of the system, so in MODE_F32R a target's own pair (dx = dy = dz =
+  // 0) contributes exactly zero without a mask.  Without it (zero softening,
+  // or a softening so small that g*m*eps^-3 overflows f32) MODE_F32R masks
+  // the pairs with a zero difference.
+  SYS_F32_SELF_FINITE = 1,
+};
 
 struct WorkItem {      // one wave: 64 consecutive targets of one system
   int32_t sys;         // index into the SysDesc table
@@ -109,6 +121,8 @@ struct Arena {         // device pointers, passed by value to every kernel
   double *px, *py, *pz; // per-part partial sums: [part * pstride + i]
   int32_t* perm;        // MODE_F32R order: slot off + k holds body
                         // off + perm[off + k] (Hilbert order)
+  int32_t* iperm;       // its inverse: body off + v sits at slot
+                        // off + iperm[off + v]
   double *hx[2], *hy[2], *hz[2];  // MODE_F32R: sx/sy/sz in slot order
   float* hg;            // MODE_F32R: gm32 in slot order
   int64_t pstride;     // cap + kPad

@@ -61,6 +61,8 @@ def _bind(lib):
         "nbody_ctx_device_info": ([vp, ctypes.c_char_p, i, ctypes.POINTER(i)], i),
         "nbody_ctx_record": ([vp, i], i),
         "nbody_ctx_elapsed": ([vp, i, i, ctypes.POINTER(ctypes.c_float)], i),
+        "nbody_ctx_kernel_timing": ([vp, i], i),
+        "nbody_ctx_kernel_times": ([vp, _D], i),
         "nbody_tuning_set_split": ([ctypes.POINTER(i), i], i),
         "nbody_tuning_get_split": ([ctypes.POINTER(i), i], i),
         "nbody_bodies_create": ([vp, i64] + [_D] * 7 + [pp], i),
@@ -280,6 +282,19 @@ class GpuContext:
         _check(load_library().nbody_ctx_elapsed(self._h, slot_from, slot_to,
                                                 ctypes.byref(ms)))
         return ms.value
+
+    def kernel_timing(self, enable: bool):
+        """Bracket every force and combine launch of this context's bodies
+        and batches with HIP events (zeroes the sums)."""
+        _check(load_library().nbody_ctx_kernel_timing(self._h, int(bool(enable))))
+
+    def kernel_times(self) -> dict:
+        """Summed launch times since kernel_timing(True): force and combine
+        milliseconds and launch counts (waits for the recorded launches)."""
+        out = np.zeros(4)
+        _check(load_library().nbody_ctx_kernel_times(self._h, _p(out)))
+        return {"force_ms": float(out[0]), "force_launches": int(out[1]),
+                "combine_ms": float(out[2]), "combine_launches": int(out[3])}
 
     def close(self):
         if getattr(self, "_h", None):

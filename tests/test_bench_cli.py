@@ -3,7 +3,6 @@ import json
 import os
 import subprocess
 import sys
-import types
 
 import numpy as np
 
@@ -21,12 +20,24 @@ def test_more_than_one_gpu_is_refused():
 
 
 def test_roofline_counts_twenty_flops_per_interaction():
-    r = bench.roofline(3.0e12, "f32", 1.5, traffic=123)
+    kt = {"force_ms": 1000.0, "force_launches": 10, "combine_ms": 20.0,
+          "combine_launches": 10}
+    # 3e12 interactions over 1 s of force launches: 60 TFLOP/s at 20 flops each
+    r = bench.roofline("f32", 3.0e12, kt, step_rate=2.9e12, traffic=123)
     assert r["bound"] == "valu" and r["unit"] == "TFLOP/s"
-    assert abs(r["achieved"] - 60.0) < 1e-9            # 3e12 * 20 / 1e12
+    assert abs(r["achieved"] - 60.0) < 1e-9
     assert r["peak"] == 157.3 and abs(r["frac"] - 60.0 / 157.3) < 1e-12
-    assert r["traffic"] == 123 and r["kernel_avg_ms"] == 1.5
-    assert bench.roofline(1.0e12, "f64", 2.0)["peak"] == 78.6
+    assert r["traffic"] == 123
+    assert r["force_kernel_avg_ms"] == 100.0 and r["combine_kernel_avg_ms"] == 2.0
+    assert abs(r["frac_of_whole_step"] - 58.0 / 157.3) < 1e-12
+    assert bench.roofline("f64", 1.0e12, kt)["peak"] == 78.6
+
+
+def test_cpu_baseline_threads_follow_the_affinity_mask():
+    threads, aff, logical = bench.host_cpus()
+    assert aff == len(os.sched_getaffinity(0))
+    assert threads == min(aff, bench.HOST_CPU_SHARE) and threads >= 1
+    assert logical == os.cpu_count()
 
 
 def test_measured_traffic_reads_the_committed_profile():
@@ -36,15 +47,17 @@ def test_measured_traffic_reads_the_committed_profile():
     assert all(1e7 < v < 1e9 for v in t.values())
 
 
-def test_default_f32_form_per_workload():
-    assert bench.default_f32("single") == "f32r"
-    assert bench.default_f32("large") == "f32r"
-    assert bench.default_f32("batched") == "f32c"
-    assert bench.default_f32("serve") == "f32c"
-    args = types.SimpleNamespace(f32="f32r")
-    assert bench.f32_forms(args) == ["f32r", "f32c"]
-    args.f32 = "f32c"
-    assert bench.f32_forms(args) == ["f32c", "f32r"]
+def test_f32_forms():
+    assert bench.other_f32("f32r") == "f32c" and bench.other_f32("f32c") == "f32r"
+    assert set(bench.PRECS) == {"f32r", "f32c", "f64"}
+
+
+def test_serving_stream_is_seeded():
+    a = bench.serve_requests(6, "plummer,star")
+    b = bench.serve_requests(6, "plummer,star")
+    assert [len(q["b"]) for q in a] == [len(q["b"]) for q in b]
+    assert all(32 <= len(q["b"]) <= 16384 and 10 <= q["steps"] <= 1000 for q in a)
+    assert all(q["dt"] in (1e-3, 5e-4) for q in a)
 
 
 def test_relative_error_helper():

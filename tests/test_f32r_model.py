@@ -17,6 +17,7 @@ tight pairs in a sparse halo, far from their wave's origin.
     while the relative form without the near path does not.
 """
 import numpy as np
+import pytest
 
 import oracle
 from test_kernel_bounds import hilbert48
@@ -119,3 +120,55 @@ def test_far_pairs_within_bound_and_accuracy():
     acc0, _ = model_accelerations(mp, P, eps * eps, near_path=False)
     e0 = np.linalg.norm(acc0 - ref, axis=1) / np.linalg.norm(ref, axis=1)
     assert e0.max() > 1e-4, e0.max()
+
+
+def f32_pair_k(gm, r2, clamp):
+    """k = g*m * rsq(r2)^3 in float32 as the F32R pair loop forms it; with
+    `clamp` the masked variant's clamp to +-FLT_MAX."""
+    with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
+        inv = F(1.0) / np.sqrt(r2.astype(F))
+        k = gm.astype(F) * ((inv * inv) * inv)
+        if clamp:
+            k = np.maximum(np.minimum(k, np.finfo(F).max), -np.finfo(F).max)
+    return k
+
+
+def self_term_finite(g, mmax, eps2):
+    """The host's rule for the unmasked F32R sweep (engine.cpp desc_of):
+    eps2 rounded to f32 is positive and g*mmax*eps2f^-1.5 < 2^120."""
+    e2f = float(F(eps2))
+    if not e2f > 0.0:
+        return False
+    inv = 1.0 / np.sqrt(e2f)
+    return abs(g) * mmax * ((inv * inv) * inv) < 2.0 ** 120
+
+
+@pytest.mark.parametrize("eps", [0.005, 1e-7, 1e-14, 0.0])
+def test_self_term_without_mask_or_with_clamp(eps):
+    """A target's own pair has dx = dy = dz = 0.  Unmasked, it adds k * 0:
+    zero while k is finite, NaN once g*m*eps^-3 overflows f32 (g*m = 1/1024
+    at eps = 1e-14 gives 1e39).  The host picks the unmasked sweep only where
+    k is finite; otherwise the clamped k times the zero difference adds
+    exactly zero, and the other pairs are unchanged, so the accelerations
+    match the oracle's at any softening."""
+    rng = np.random.default_rng(7)
+    n = 256
+    m = np.full(n, 1.0 / 1024)
+    P = rng.standard_normal((n, 3))
+    eps2 = eps * eps
+    d = (P[None, :, :] - P[:, None, :]).astype(F)
+    r2 = (d * d).sum(2, dtype=F) + F(eps2)
+    nomask = self_term_finite(1.0, m.max(), eps2)
+    k = f32_pair_k(np.broadcast_to(m[None, :], r2.shape), r2, clamp=not nomask)
+    with np.errstate(invalid="ignore"):
+        acc = np.einsum("ij,ijk->ik", k.astype(np.float64), d.astype(np.float64))
+    ref = np.stack(oracle.newton(m, P[:, 0].copy(), P[:, 1].copy(), P[:, 2].copy(),
+                                 1.0, eps), 1)
+    e = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.isfinite(acc).all()
+    assert e.max() <= 1e-5, e.max()
+    # the unmasked sweep would produce NaN exactly where the host refuses it
+    k_plain = f32_pair_k(np.broadcast_to(m[None, :], r2.shape), r2, clamp=False)
+    with np.errstate(invalid="ignore"):
+        self_nan = np.isnan(np.diagonal(k_plain) * F(0)).any()
+    assert self_nan == (not nomask), (eps, nomask)

@@ -453,9 +453,10 @@ def test_coincident_bodies_unsoftened(gpu):
     """Two distinct bodies at one point, eps = 0.  F64_STRICT, F64 and F32C
     follow the reference (1/sqrt(0) = inf, inf * 0 = NaN): the pair's two
     bodies get NaN; the others get the oracle's bits (strict) or agree within
-    the form's tolerance.  F32R drops pairs at r2 == 0 (that is how it
-    excludes a target's own term), so every body gets the finite sum over
-    its other sources."""
+    the form's tolerance.  F32R clamps each pair's k to the f32 range where
+    the softening does not keep the self term finite (that is how it drops a
+    target's own term), so a zero difference adds zero and every body gets
+    the finite sum over its other sources."""
     m = np.array([1.0, 2.0, 3.0, 4.0, 5.0])
     x = np.array([0.0, 0.0, 1.0, 0.0, 0.0])
     y = np.array([0.0, 0.0, 0.0, 2.0, 0.0])
@@ -484,6 +485,22 @@ def test_coincident_bodies_unsoftened(gpu):
     assert e.max() <= 1e-5, e.max()
 
 
+@pytest.mark.parametrize("eps", [1e-14, 1e-9])
+def test_tiny_softening_f32_forms(gpu, eps):
+    """Softening so small that an unmasked self term overflows f32: g*m =
+    1/2048 and eps = 1e-14 give g*m*eps^-3 = 5e38 > FLT_MAX, so k * 0 for a
+    target's own pair would be NaN.  F32R then takes its masked sweep and
+    F32C masks the target by index; both stay within 1e-5 of the oracle.  At
+    eps = 1e-9 the self term is finite and F32R runs unmasked."""
+    b = nb.plummer(2048, 41)
+    ref = oracle.newton_parallel(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, eps)
+    for prec in (nb.Precision.F32R, nb.Precision.F32C):
+        got = gpu_acc(gpu, b, 1.0, eps, prec)
+        assert all(np.isfinite(a).all() for a in got), prec.name
+        e = rel_err(got, ref)
+        assert e.max() <= 1e-5, (prec.name, e.max())
+
+
 @pytest.mark.parametrize("shift", [1e4, -3e6])
 def test_system_far_from_the_origin(gpu, shift):
     """A 4096-body Plummer sphere moved far from the coordinate origin: the
@@ -499,3 +516,44 @@ def test_system_far_from_the_origin(gpu, shift):
         e = rel_err(gpu_acc(gpu, b, 1.0, 0.005, prec), ref)
         print(f"shift {shift:g} {prec.name}: max {e.max():.3e}", flush=True)
         assert e.max() <= tol, prec.name
+
+
+def test_kernel_timing_counts_and_bits(gpu):
+    """Launch timing brackets every force and combine launch with events:
+    five velocity-Verlet steps are five force launches (one combine each when
+    the system is split), their summed times fit inside the events around the
+    steps, and a batch stepped with timing on (launched without its graph)
+    keeps the bits of the graph-stepped one."""
+    b = nb.plummer(4096, 3)
+    db = nb.DeviceBodies(gpu, b)
+    grav = nb.GpuNewtonParallelGravity(1.0, 1e-4, nb.Precision.F32R)
+    integ = nb.GpuVelocityVerletIntegrator(1e-3)
+    integ.step(gpu, db, grav, 1)   # primes the carried acceleration
+    gpu.synchronize()
+    gpu.kernel_timing(True)
+    gpu.record(0)
+    integ.step(gpu, db, grav, 5)
+    gpu.record(1)
+    total = gpu.elapsed_ms(0, 1)
+    kt = gpu.kernel_times()
+    assert kt["force_launches"] == 5
+    assert kt["combine_launches"] == (5 if nb.tuning_split()[12] > 1 else 0)
+    assert 0.0 < kt["force_ms"] + kt["combine_ms"] <= total * 1.001
+    assert gpu.kernel_times() == kt           # harvesting twice adds nothing
+    gpu.kernel_timing(False)
+    assert gpu.kernel_times()["force_launches"] == 0
+    db.close()
+    systems = [nb.star_system(n, 70 + n) for n in (100, 700, 3000)]
+    out = []
+    for timed in (False, True):
+        bt = nb.Batch(gpu, systems, [0, 1, 2], 1e-3, 1.0, 1e-6, nb.Precision.F32R)
+        gpu.kernel_timing(timed)
+        bt.step(4)
+        if timed:
+            kt = gpu.kernel_times()
+            assert kt["force_launches"] == 4 * 4 + 1   # 4 RK4 rounds per step, VV prime
+        gpu.kernel_timing(False)
+        out.append(bt.download())
+        bt.close()
+    for a, c in zip(*out):
+        assert np.array_equal(a, c)
