@@ -152,6 +152,30 @@ int nbody_calculate_accelerations(nbody_ctx *ctx, nbody_bodies *b,
                                   int precision, double *acc_x,
                                   double *acc_y, double *acc_z);
 
+/* ---- gravity on caller-owned device arrays, allocating nothing.  No
+ * counterpart in the reference: this is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  masses, pos_* (inputs) and acc_*
+ * (outputs) are DEVICE pointers to n doubles each, owned by the caller.
+ * The evaluation is queued on the context's stream and returns without
+ * waiting; its results are the bits nbody_calculate_accelerations gives
+ * for the same bodies and arguments (direct sum).  The library's working
+ * memory (work lists, g*m, a padded position snapshot, split partial sums)
+ * is one static device array of 64 MiB per GPU, held until the process
+ * ends: n is at most 65536 (NBODY_E_INVALID beyond), and precision is
+ * NBODY_PRECISION_F64_STRICT, F64 or F32C (F32R: NBODY_E_UNSUPPORTED, its
+ * body ordering needs temporary memory).  Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (contexts on different streams)
+ * race: order them. */
+int nbody_calculate_accelerations_noalloc(nbody_ctx *ctx, int64_t n,
+                                          const double *masses,
+                                          const double *pos_x,
+                                          const double *pos_y,
+                                          const double *pos_z, double g,
+                                          double eps2, int precision,
+                                          double *acc_x, double *acc_y,
+                                          double *acc_z);
+
 /* ---- integrator: replaces GpuIntegrator::step
  * (src/integrators/gpu/gpu_integrator.rs:8-15; src/integrators/gpu/euler.rs:17-43)
  * called nsteps times.  All three integrators are offered (the reference GPU

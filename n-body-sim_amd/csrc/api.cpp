@@ -13,6 +13,7 @@
 #include "engine.h"
 #include "formats.h"
 #include "kernels.h"
+#include "noalloc.h"
 #include "server.h"
 #include "tuning.h"
 
@@ -165,6 +166,8 @@ int nbody_ctx_create(int device, nbody_ctx** out) {
     c->cus = prop.multiProcessorCount;
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking),
               "hipStreamCreate");
+    // the allocation-free entry point's scratch address on this device
+    nbody::noalloc_init(device);
     // init check: one probe launch must run and write its marker
     int* d = nullptr;
     int h = 0;
@@ -377,6 +380,20 @@ int nbody_calculate_accelerations(nbody_ctx* ctx, nbody_bodies* b,
     p.eps2 = eps2;
     b->eng->set_params(b->id, p);
     b->eng->accelerations(b->id, acc_x, acc_y, acc_z);
+  });
+}
+
+int nbody_calculate_accelerations_noalloc(
+    nbody_ctx* ctx, int64_t n, const double* masses, const double* pos_x,
+    const double* pos_y, const double* pos_z, double g, double eps2,
+    int precision, double* acc_x, double* acc_y, double* acc_z) {
+  return guarded([&] {
+    require(ctx != nullptr, NBODY_E_INVALID, "null context");
+    const int mode = mode_of(precision);
+    ctx->use();
+    nbody::noalloc_accelerations(ctx->device, ctx->stream, mode, n, masses,
+                                 pos_x, pos_y, pos_z, g, eps2, acc_x, acc_y,
+                                 acc_z);
   });
 }
 

@@ -67,7 +67,6 @@ constexpr int64_t kAlign = 64;  // system ranges start on a wave boundary
 int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 }  // namespace
 
-static int split_for(int64_t n);
 
 Engine::Engine(int device, hipStream_t stream, int mode, int64_t capacity,
                int max_systems)
@@ -296,7 +295,7 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
   s.p = p;
   s.off = off;
   s.span = need;
-  s.split = std::min(split_for(p.n), split_cap_);
+  s.split = std::min(split_for_size(p.n), split_cap_);
   // centre of mass and its velocity, f64 in index order (the centre of the
   // MODE_F32R ordering box, advanced by vcom * elapsed time), and the
   // largest coordinate distance from it (the box size)
@@ -478,7 +477,7 @@ int Engine::split_of(int id) const {
 
 // The split a system gets for its lifetime, fixed when it is uploaded so the
 // host work lists and the device descriptor can never disagree.
-static int split_for(int64_t n) {
+int split_for_size(int64_t n) {
   int k = tuning_split(n);
   while (k > 1 && n / k < 64) --k;
   return std::max(1, std::min(k, kMaxSplit));

@@ -57,6 +57,11 @@ class KernelTimer {
   size_t used_ = 0;  // events recorded since the last harvest
 };
 
+// Source-range parts of a system of n bodies (tuning table by floor(log2 n),
+// clamped so every part keeps at least 64 sources, at most kMaxSplit); the
+// engine clamps it further by its split cap.
+int split_for_size(int64_t n);
+
 struct SystemParams {
   int64_t n = 0;
   int integ = INTEG_EULER;

@@ -72,6 +72,8 @@ def _bind(lib):
         "nbody_bodies_set_theta": ([vp, d], i),
         "nbody_bodies_tree_nodes": ([vp, ctypes.POINTER(i64)], i),
         "nbody_calculate_accelerations": ([vp, vp, i, d, d, i, _D, _D, _D], i),
+        "nbody_calculate_accelerations_noalloc": ([vp, i64, vp, vp, vp, vp, d, d, i,
+                                                   vp, vp, vp], i),
         "nbody_step": ([vp, vp, i, i, i, d, d, d, i64], i),
         "nbody_step_record": ([vp, vp, i, i, i, d, d, d, i64, _D, _D, _D, i64], i),
         "nbody_energy_momentum": ([vp, vp, d, d, _D], i),
