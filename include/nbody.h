@@ -313,10 +313,11 @@ int nbody_server_poll(nbody_server *s, int timeout_ms, nbody_event *ev,
                       double *pos_x, double *pos_y, double *pos_z,
                       double *vel_x, double *vel_y, double *vel_z,
                       int64_t capacity);
-/* Cancels a queued request (its CANCELLED event is queued at once) or a
- * resident one (retired at the next epoch end with a CANCELLED event, unless
- * its DONE was already queued).  NBODY_E_NOTFOUND for an id that is not
- * queued or resident, including a request already retired after its DONE. */
+/* Cancels a queued request (its CANCELLED event is queued at once) or one
+ * being admitted or resident (retired at the next epoch end with a
+ * CANCELLED event, unless its DONE was already queued).  NBODY_E_NOTFOUND
+ * for an id that is not queued, being admitted or resident, including a
+ * request already retired after its DONE. */
 int nbody_server_cancel(nbody_server *s, uint64_t request_id);
 /* Counters: out[0] submitted, [1] admitted, [2] completed, [3] resident,
  * [4] queued, [5] shared steps run, [6] pair interactions evaluated,

@@ -244,18 +244,6 @@ int64_t Engine::steps_done(int id) const {
   return sys_[id].steps;
 }
 
-int64_t Engine::take_range(std::map<int64_t, int64_t>& free, int64_t n) {
-  const int64_t need = round_up(n, kAlign);
-  auto it = free.begin();
-  for (; it != free.end(); ++it)
-    if (it->second >= need) break;
-  if (it == free.end()) return -1;
-  const int64_t off = it->first, len = it->second;
-  free.erase(it);
-  if (len > need) free[off + need] = len - need;
-  return off;
-}
-
 static void validate(const SystemParams& p) {
   if (p.n <= 0) throw Error(NBODY_E_INVALID, "system must have n >= 1 bodies");
   if (p.n > INT32_MAX) throw Error(NBODY_E_INVALID, "n too large");

@@ -108,7 +108,17 @@ class Engine {
   // its offset, or -1 (leaving `free` as it was) when none holds n.
   // Replaying it on a copy tells which of several uploads will fit.
   std::map<int64_t, int64_t> free_ranges() const { return free_; }
-  static int64_t take_range(std::map<int64_t, int64_t>& free, int64_t n);
+  static int64_t take_range(std::map<int64_t, int64_t>& free, int64_t n) {
+    const int64_t need = (n + kWave - 1) / kWave * kWave;
+    auto it = free.begin();
+    for (; it != free.end(); ++it)
+      if (it->second >= need) break;
+    if (it == free.end()) return -1;
+    const int64_t off = it->first, len = it->second;
+    free.erase(it);
+    if (len > need) free[off + need] = len - need;
+    return off;
+  }
   int64_t capacity() const { return cap_; }
   int nsystems() const;
   const SystemParams& params(int id) const;

@@ -3,8 +3,9 @@
 // One worker thread owns the engine.  Requests wait in a FIFO until their
 // device footprint (a contiguous arena range plus a table slot) is free;
 // admission is first-fit in arrival order, and a request that does not fit
-// may be bypassed by later ones at most kMaxBypass times before admission
-// stops behind it (no starvation).  All resident systems advance together in
+// may be bypassed by later ones (each later request admitted past it counts
+// once) at most kMaxBypass times before admission stops behind it (no
+// starvation).  All resident systems advance together in
 // epochs of snapshot_every steps; a system whose request ends inside an
 // epoch stops stepping at its last step while the others go on, reports
 // DONE at that point, and is retired at the epoch's end; the epoch is cut at
@@ -89,6 +90,27 @@ int mode_of(int precision) {
 
 int64_t evals_per_step(int integ) { return integ == INTEG_RK4 ? 4 : 1; }
 }  // namespace
+
+std::vector<size_t> admission_pass(const std::vector<int64_t>& sizes,
+                                   std::vector<int>& bypassed,
+                                   std::map<int64_t, int64_t> free,
+                                   int free_slots, int max_bypass) {
+  std::vector<size_t> take;
+  std::vector<size_t> passed;  // positions that did not fit
+  for (size_t i = 0; i < sizes.size() && free_slots > 0; ++i) {
+    if (Engine::take_range(free, sizes[i]) < 0) {
+      passed.push_back(i);
+      continue;
+    }
+    bool starving = false;
+    for (size_t w : passed) starving = starving || bypassed[w] >= max_bypass;
+    if (starving) break;
+    for (size_t w : passed) ++bypassed[w];
+    --free_slots;
+    take.push_back(i);
+  }
+  return take;
+}
 
 class Server {
  public:
@@ -216,6 +238,12 @@ class Server {
         queue_.erase(it);
         return;
       }
+    // taken from the queue, its upload still running: cancelled once resident
+    auto ad = admitting_.find(id);
+    if (ad != admitting_.end()) {
+      ad->second = true;
+      return;
+    }
     auto a = active_.find(id);
     if (a == active_.end()) throw Error(NBODY_E_NOTFOUND, "unknown request id");
     a->second.cancel = true;
@@ -257,22 +285,29 @@ class Server {
   // is replayed on a copy of the arena's free ranges with add()'s own rule,
   // in admission order, so every admitted request gets a range when it is
   // uploaded (ranges taken by earlier admissions of the same pass are gone).
+  // Admitting a request bypasses every earlier one that did not fit; once one
+  // of those has been bypassed kMaxBypass times, admission stops behind it.
+  // Admitted ids stay in admitting_ until they are resident, so a cancel
+  // that arrives during the upload is kept.
   std::vector<Request> take_admissible() {
-    std::vector<Request> out;
-    std::map<int64_t, int64_t> free = eng_->free_ranges();
-    int free_slots = cfg_.max_systems - eng_->nsystems();
-    bool blocked = false;
-    for (auto it = queue_.begin(); it != queue_.end() && free_slots > 0;) {
-      if (!blocked && Engine::take_range(free, it->p.n) >= 0) {
-        --free_slots;
-        out.push_back(std::move(*it));
-        it = queue_.erase(it);
-        continue;
-      }
-      if (++it->bypassed > kMaxBypass) blocked = true;
-      if (blocked) break;
-      ++it;
+    std::vector<int64_t> sizes;
+    std::vector<int> bypassed;
+    for (const Request& q : queue_) {
+      sizes.push_back(q.p.n);
+      bypassed.push_back(q.bypassed);
     }
+    const std::vector<size_t> take =
+        admission_pass(sizes, bypassed, eng_->free_ranges(),
+                       cfg_.max_systems - eng_->nsystems(), kMaxBypass);
+    for (size_t k = 0; k < queue_.size(); ++k) queue_[k].bypassed = bypassed[k];
+    std::vector<Request> out;
+    for (size_t k : take) {
+      admitting_[queue_[k].id] = false;
+      out.push_back(std::move(queue_[k]));
+    }
+    // erase from the back so earlier positions stay valid
+    for (size_t q = take.size(); q-- > 0;)
+      queue_.erase(queue_.begin() + (std::ptrdiff_t)take[q]);
     return out;
   }
 
@@ -305,6 +340,7 @@ class Server {
           Event e;
           fill(e.ev, NBODY_EVENT_FAILED, q, 0, a.t_admit, ex.what());
           std::lock_guard<std::mutex> lk(mu_);
+          admitting_.erase(q.id);
           push_event(std::move(e));
           ++failed_;
           continue;
@@ -315,6 +351,11 @@ class Server {
         std::lock_guard<std::mutex> lk(mu_);
         ++admitted_;
         const uint64_t id = a.req.id;
+        auto ad = admitting_.find(id);
+        if (ad != admitting_.end()) {
+          a.cancel = ad->second;  // cancelled during the upload
+          admitting_.erase(ad);
+        }
         active_.emplace(id, std::move(a));
       }
       // retire cancelled and finished requests; the epoch is snapshot_every
@@ -554,6 +595,8 @@ class Server {
   std::mutex mu_;
   std::condition_variable cv_work_, cv_events_;
   std::deque<Request> queue_;
+  std::map<uint64_t, bool> admitting_;  // taken from the queue, uploading:
+                                        // id -> cancel requested
   std::map<uint64_t, Active> active_;
   std::deque<Event> events_;
   size_t queued_bytes_ = 0;
