@@ -186,6 +186,37 @@ int nbody_step(nbody_ctx *ctx, nbody_bodies *b, int integrator,
                int gravity_method, int precision, double dt, double g,
                double eps2, int64_t nsteps);
 
+/* ---- one integrator stage on the device acceleration arrays: replaces
+ * the reference's euler_step launch (src/integrators/gpu/euler.rs:26-41,
+ * cuda/kernels.cu:54-77) and adds the stages of velocity Verlet and RK4, so
+ * that a GpuIntegrator can step through the unchanged GpuGravity trait
+ * (src/gravity.rs:30-36): gravity.calculate_accelerations (here
+ * nbody_calculate_accelerations, which leaves a(x) in the device arrays),
+ * then the stage, in the reference's order:
+ *   euler:           F, EULER
+ *   velocity-verlet: F, VV_DRIFT, F, VV_KICK  (two evaluations per step,
+ *                    src/integrators/cpu/velocity_verlet.rs:76,:90)
+ *   runge-kutta:     F, RK4_1, F, RK4_2, F, RK4_3, F, RK4_4
+ * RK4_1..RK4_3 move the bodies to the next stage's evaluation point (the
+ * step's start is kept on the device) and RK4_4 writes the step's result.
+ * The expression trees are those of nbody_step, so a step run this way gives
+ * nbody_step's bits for the same precision.  (Except velocity Verlet in
+ * NBODY_PRECISION_F32R at a step where the bodies are re-sorted, every 32
+ * steps of their own count: the re-evaluated a(n) is summed in the new
+ * order, where nbody_step carries the value summed in the old one; the two
+ * agree to rounding.)  dt is the step (the same for
+ * every stage of a step).  Asynchronous; NBODY_E_INVALID for an unknown
+ * stage or a non-finite dt. */
+#define NBODY_STAGE_EULER 0     /* v += a dt; x += v dt */
+#define NBODY_STAGE_VV_DRIFT 1  /* keep a as a_n; x += (v dt) + ((0.5 a) dt^2) */
+#define NBODY_STAGE_VV_KICK 2   /* v += (0.5 (a_n + a)) dt */
+#define NBODY_STAGE_RK4_1 3
+#define NBODY_STAGE_RK4_2 4
+#define NBODY_STAGE_RK4_3 5
+#define NBODY_STAGE_RK4_4 6
+int nbody_integrate_stage(nbody_ctx *ctx, nbody_bodies *b, int stage,
+                          double dt);
+
 /* ---- stepping with a snapshot after every step: replaces the loop of
  * SimulationRunner::run (src/simulation_runner.rs:28-52) over
  * GpuSimulation::step and ::snapshot (src/simulation/gpu_simulation.rs:45-65),

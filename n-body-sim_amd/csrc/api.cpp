@@ -86,6 +86,13 @@ struct nbody_batch {
   int64_t total = 0;
 };
 
+static_assert(NBODY_STAGE_EULER == nbody::OP_EULER &&
+                  NBODY_STAGE_VV_DRIFT == nbody::OP_VV_DRIFT &&
+                  NBODY_STAGE_VV_KICK == nbody::OP_VV_KICK &&
+                  NBODY_STAGE_RK4_1 == nbody::OP_RK4_1 &&
+                  NBODY_STAGE_RK4_4 == nbody::OP_RK4_4,
+              "public stage ids are the kernel's");
+
 namespace {
 int mode_of(int precision) {
   switch (precision) {
@@ -394,6 +401,21 @@ int nbody_calculate_accelerations_noalloc(
     nbody::noalloc_accelerations(ctx->device, ctx->stream, mode, n, masses,
                                  pos_x, pos_y, pos_z, g, eps2, acc_x, acc_y,
                                  acc_z);
+  });
+}
+
+int nbody_integrate_stage(nbody_ctx* ctx, nbody_bodies* b, int stage,
+                          double dt) {
+  return guarded([&] {
+    require(ctx && b && b->ctx == ctx, NBODY_E_INVALID, "bad context/bodies");
+    require(stage >= NBODY_STAGE_EULER && stage <= NBODY_STAGE_RK4_4,
+            NBODY_E_INVALID, "unknown integrator stage");
+    check_dt(dt);
+    ctx->use();
+    nbody::SystemParams p = b->eng->params(b->id);
+    p.dt = dt;
+    b->eng->set_params(b->id, p);
+    b->eng->integrate_stage(b->id, stage);
   });
 }
 

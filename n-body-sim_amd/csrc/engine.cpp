@@ -153,6 +153,7 @@ size_t Engine::carve(char* base) {
   A_.ay = D();
   A_.az = D();
   for (int q = 0; q < 9; ++q) A_.rk[q] = D();
+  for (int q = 0; q < 3; ++q) A_.xs[q] = D();
   A_.phi = D();
   A_.px = D(kMaxSplit);
   A_.py = D(kMaxSplit);
@@ -1048,6 +1049,28 @@ void Engine::accelerations(int id, double* ax, double* ay, double* az) {
                 "hipMemcpy(acc)");
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   }
+}
+
+void Engine::integrate_stage(int id, int op) {
+  check_id(id);
+  if (op < OP_EULER || op > OP_RK4_4)
+    throw Error(NBODY_E_INVALID, "unknown integrator stage");
+  Sys& s = sys_[id];
+  upload_table();
+  std::vector<WorkItem> items;
+  tiles_of(id, STAGE_ACC, s.cur, items);
+  validate_items(items);
+  ensure_items(items.size());
+  hip_check(hipMemcpyAsync(d_items_, items.data(), sizeof(WorkItem) * items.size(),
+                           hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(work list)");
+  hip_check(launch_stage(d_arena_, d_items_, (int)items.size(), d_sys_, op, stream_),
+            "stage launch");
+  // the pageable host list must not change before the copy is consumed
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  s.prepared = false;    // positions moved: rebuild the snapshot
+  s.vv_primed = false;   // a carried acceleration no longer applies
+  if (op == OP_EULER || op == OP_VV_KICK || op == OP_RK4_4) ++s.steps;
 }
 
 void Engine::diagnostics(const std::vector<int>& ids, double* out) {

@@ -164,6 +164,10 @@ class Engine {
   // F(x) of one system into the device acceleration arrays (and optionally
   // to the host).  Drops a velocity-Verlet carry of that system.
   void accelerations(int id, double* ax, double* ay, double* az);
+  // One integrator stage (StageOp) of one system on its acceleration arrays
+  // (the result of accelerations()), with the system's dt.  Stages that move
+  // positions invalidate the snapshot; the stages that end a step count it.
+  void integrate_stage(int id, int op);
   // Per-system diagnostics, DIAG_COUNT doubles per listed system.
   void diagnostics(const std::vector<int>& ids, double* out);
   // Downloads the master state (any pointer may be null).  Synchronous.

@@ -20,6 +20,10 @@ hipError_t launch_potential(const Arena* A, const WorkItem* items, int nitems,
 // Snapshot (and gm) rebuild from the master state.
 hipError_t launch_prepare(const Arena* A, const WorkItem* items, int nitems,
                           const SysDesc* sys, hipStream_t s);
+// One integrator stage (StageOp) on the acceleration arrays of the listed
+// tiles' bodies.
+hipError_t launch_stage(const Arena* A, const WorkItem* items, int nitems,
+                        const SysDesc* sys, int op, hipStream_t s);
 // Per-system diagnostics (Diag layout), one workgroup per listed system.
 hipError_t launch_diag(const Arena* A, const SysDesc* sys, const int32_t* list,
                        int nlist, double* out, hipStream_t s);

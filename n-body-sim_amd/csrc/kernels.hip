@@ -685,6 +685,100 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const Arena* Ap,
   A->sz[d][i] = z;
 }
 
+// One integrator stage on the acceleration arrays (StageOp), with the
+// reference's expression trees (src/integrators/cpu/euler.rs:69-80,
+// velocity_verlet.rs:82-101, runge_kutta.rs:149-281) written exactly as the
+// fused epilogue writes them, so a step run stage by stage around separate
+// force evaluations gives the fused step's bits.  Staged RK4 moves the
+// master positions to each stage's evaluation point and keeps the step's
+// start in xs.
+__global__ __launch_bounds__(kBlock) void k_stage(const Arena* Ap,
+                                                  const WorkItem* items,
+                                                  int nitems,
+                                                  const SysDesc* sysd,
+                                                  int op) {
+#pragma clang fp contract(off)
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int idx = (int)blockIdx.x * kWavesPerBlock + wave;
+  if (idx >= nitems) return;
+  const WorkItem it = load_uniform(items + idx);
+  const CArena* A = (const CArena*)Ap;
+  const CSys* sd = (const CSys*)(sysd + it.sys);
+  const int64_t end = sd->off + sd->n;
+  const int64_t i = sd->off + (int64_t)it.tile * kWave +
+                    (int64_t)(threadIdx.x & (kWave - 1));
+  if (i >= end) return;
+  const double dt = sd->dt;
+  const double ax = A->ax[i], ay = A->ay[i], az = A->az[i];
+  double* X[3] = {A->x, A->y, A->z};
+  double* V[3] = {A->vx, A->vy, A->vz};
+  const double a[3] = {ax, ay, az};
+  switch (op) {
+    case OP_EULER:
+      for (int c = 0; c < 3; ++c) {
+        const double v = V[c][i] + a[c] * dt;
+        V[c][i] = v;
+        X[c][i] = X[c][i] + v * dt;
+      }
+      break;
+    case OP_VV_DRIFT: {
+      const double dt2 = dt * dt;
+      for (int c = 0; c < 3; ++c) {
+        A->rk[c][i] = a[c];
+        X[c][i] = X[c][i] + ((V[c][i] * dt) + ((0.5 * a[c]) * dt2));
+      }
+      break;
+    }
+    case OP_VV_KICK:
+      for (int c = 0; c < 3; ++c)
+        V[c][i] = V[c][i] + ((0.5 * (A->rk[c][i] + a[c])) * dt);
+      break;
+    case OP_RK4_1: {
+      const double h = dt / 2.0;
+      for (int c = 0; c < 3; ++c) {
+        const double v = V[c][i], x0 = X[c][i];
+        const double k2 = v + h * a[c];
+        A->xs[c][i] = x0;
+        A->rk[c][i] = v + (2.0 * k2);
+        A->rk[3 + c][i] = a[c];
+        A->rk[6 + c][i] = k2;
+        X[c][i] = x0 + h * v;
+      }
+      break;
+    }
+    case OP_RK4_2: {
+      const double h = dt / 2.0;
+      for (int c = 0; c < 3; ++c) {
+        const double k3 = V[c][i] + h * a[c];
+        const double k2 = A->rk[6 + c][i];
+        A->rk[c][i] = A->rk[c][i] + (2.0 * k3);
+        A->rk[3 + c][i] = A->rk[3 + c][i] + (2.0 * a[c]);
+        A->rk[6 + c][i] = k3;
+        X[c][i] = A->xs[c][i] + h * k2;
+      }
+      break;
+    }
+    case OP_RK4_3:
+      for (int c = 0; c < 3; ++c) {
+        const double k3 = A->rk[6 + c][i];
+        A->rk[c][i] = A->rk[c][i] + (V[c][i] + dt * a[c]);
+        A->rk[3 + c][i] = A->rk[3 + c][i] + (2.0 * a[c]);
+        X[c][i] = A->xs[c][i] + dt * k3;
+      }
+      break;
+    case OP_RK4_4: {
+      const double s6 = dt / 6.0;
+      for (int c = 0; c < 3; ++c) {
+        X[c][i] = A->xs[c][i] + s6 * A->rk[c][i];
+        V[c][i] = V[c][i] + s6 * (A->rk[3 + c][i] + a[c]);
+      }
+      break;
+    }
+    default:
+      break;
+  }
+}
+
 // Per-system f64 diagnostics, one workgroup per listed system, fixed
 // summation order (thread t takes bodies t, t+256, ...; then a fixed LDS
 // tree), so results are reproducible bit for bit.  Needs phi from a
@@ -974,6 +1068,14 @@ hipError_t launch_prepare(const Arena* A, const WorkItem* items, int nitems,
   if (nitems <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_prepare, dim3(blocks_for(nitems)), dim3(kBlock), 0, s,
                      A, items, nitems, sys);
+  return hipGetLastError();
+}
+
+hipError_t launch_stage(const Arena* A, const WorkItem* items, int nitems,
+                        const SysDesc* sys, int op, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stage, dim3(blocks_for(nitems)), dim3(kBlock), 0, s, A,
+                     items, nitems, sys, op);
   return hipGetLastError();
 }
 

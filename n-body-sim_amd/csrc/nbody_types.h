@@ -11,6 +11,7 @@
 //   ax, ay, az                    carried acceleration (velocity Verlet) or the
 //                                 output of a plain force evaluation
 //   rk[9]                         RK4 running sums and next-stage velocity
+//   xs[3]                         staged RK4: positions at the step's start
 //   phi                           per-body potential (energy diagnostics)
 //   px/py/pz[kMaxSplit]           partial sums of split source ranges
 //                                 (body order in every mode)
@@ -117,6 +118,7 @@ struct Arena {         // device pointers, passed by value to every kernel
   double *sx[2], *sy[2], *sz[2];
   double *ax, *ay, *az;
   double *rk[9];
+  double *xs[3];        // staged RK4: positions at the start of the step
   double *phi;
   double *px, *py, *pz; // per-part partial sums: [part * pstride + i]
   int32_t* perm;        // MODE_F32R order: slot off + k holds body
@@ -136,6 +138,19 @@ struct SortSys {
   int32_t n, base;     // bodies; first slot in the pass's key arrays
   double ox, oy, oz;   // quantisation origin (the system's moving centre)
   double inv;          // 1 / box width
+};
+
+// One integrator stage applied on its own to the acceleration arrays (ax,
+// ay, az), for callers that evaluate the forces separately (the public
+// NBODY_STAGE_* values).
+enum StageOp : int32_t {
+  OP_EULER = 0,     // v += a*dt; x += v*dt; one step done
+  OP_VV_DRIFT = 1,  // keep a as a_n (rk[0..2]); x += (v*dt) + ((0.5*a)*dt2)
+  OP_VV_KICK = 2,   // v += (0.5*(a_n + a))*dt; one step done
+  OP_RK4_1 = 3,     // after a1 = F(x0): xs <- x0, sums, x <- x0 + h*v
+  OP_RK4_2 = 4,     // after a2: x <- x0 + h*k2r
+  OP_RK4_3 = 5,     // after a3: x <- x0 + dt*k3r
+  OP_RK4_4 = 6,     // after a4: x, v <- the step's result; one step done
 };
 
 // Per-system diagnostics written by the reduction kernel (9 doubles).

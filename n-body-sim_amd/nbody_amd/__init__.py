@@ -75,6 +75,7 @@ def _bind(lib):
         "nbody_calculate_accelerations_noalloc": ([vp, i64, vp, vp, vp, vp, d, d, i,
                                                    vp, vp, vp], i),
         "nbody_step": ([vp, vp, i, i, i, d, d, d, i64], i),
+        "nbody_integrate_stage": ([vp, vp, i, d], i),
         "nbody_step_record": ([vp, vp, i, i, i, d, d, d, i64, _D, _D, _D, i64], i),
         "nbody_energy_momentum": ([vp, vp, d, d, _D], i),
         "nbody_scenario_plummer": ([i64, u64, d, d, d] + [_D] * 7, i),
@@ -414,6 +415,39 @@ class GpuIntegrator:
 
 class GpuEulerIntegrator(GpuIntegrator):
     integrator = IntegratorMethod.EULER
+
+
+# NBODY_STAGE_* (include/nbody.h)
+STAGE_EULER, STAGE_VV_DRIFT, STAGE_VV_KICK = 0, 1, 2
+STAGE_RK4 = (3, 4, 5, 6)
+
+
+class GpuStagedIntegrator:
+    """A GpuIntegrator that steps through the gravity trait alone, as the
+    reference's GpuEulerIntegrator does (src/integrators/gpu/euler.rs:17-43):
+    gravity.calculate_accelerations(gpu, bodies) leaves a(x) in the device
+    arrays, then nbody_integrate_stage applies the stage.  Velocity Verlet
+    evaluates twice per step and RK4 four times, in the reference's order;
+    the results are nbody_step's bits (velocity Verlet in F32R: to rounding
+    at the steps where the bodies are re-sorted)."""
+
+    SEQUENCE = {
+        IntegratorMethod.EULER: (STAGE_EULER,),
+        IntegratorMethod.VELOCITY_VERLET: (STAGE_VV_DRIFT, STAGE_VV_KICK),
+        IntegratorMethod.RUNGE_KUTTA: STAGE_RK4,
+    }
+
+    def __init__(self, integrator: IntegratorMethod, dt: float):
+        self.integrator = IntegratorMethod(integrator)
+        self.dt = float(dt)
+
+    def step(self, gpu: GpuContext, bodies: DeviceBodies, gravity, nsteps: int = 1):
+        lib = load_library()
+        for _ in range(int(nsteps)):
+            for stage in self.SEQUENCE[self.integrator]:
+                gravity.calculate_accelerations(gpu, bodies, download=False)
+                _check(lib.nbody_integrate_stage(gpu.handle, bodies.handle, stage,
+                                                 self.dt))
 
 
 class GpuVelocityVerletIntegrator(GpuIntegrator):

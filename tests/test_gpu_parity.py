@@ -557,3 +557,52 @@ def test_kernel_timing_counts_and_bits(gpu):
         bt.close()
     for a, c in zip(*out):
         assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("prec", [nb.Precision.F64_STRICT, nb.Precision.F64,
+                                  nb.Precision.F32C, nb.Precision.F32R])
+@pytest.mark.parametrize("integ", list(nb.IntegratorMethod))
+def test_staged_integrator_equals_fused_step(gpu, integ, prec):
+    """The trait-preserving path (a gravity evaluation into the device
+    arrays, then nbody_integrate_stage, in the reference's evaluation order:
+    velocity Verlet twice per step, RK4 four times) gives nbody_step's bits
+    for every integrator and precision over 40 steps; in strict f64 both
+    equal the oracle's trajectory.  One exception: F32R re-sorts its bodies
+    every 32 steps of a system's own count, and velocity Verlet re-evaluates
+    a(n) at the start of each step as the reference does, so at step 32 the
+    staged path sums a(n) in the new order where the fused path carried the
+    value summed in the old one; there the two agree to rounding."""
+    b = nb.plummer(1500, 77)
+    g, eps, dt, k = 1.0, 0.01, 1e-3, 40
+    grav = nb.GpuNewtonParallelGravity(g, eps * eps, prec)
+    out = []
+    for staged in (False, True):
+        db = nb.DeviceBodies(gpu, b)
+        if staged:
+            nb.GpuStagedIntegrator(integ, dt).step(gpu, db, grav, k)
+        else:
+            integ.gpu_create(dt).step(gpu, db, grav, k)
+        out.append(db.download())
+        db.close()
+    for f in ("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z"):
+        a, c = getattr(out[0], f), getattr(out[1], f)
+        if prec == nb.Precision.F32R and integ == nb.IntegratorMethod.VELOCITY_VERLET:
+            assert np.abs(a - c).max() <= 1e-10 * max(1.0, np.abs(a).max()), (f, np.abs(a - c).max())
+        else:
+            assert np.array_equal(a, c), (integ.name, f)
+    if prec == nb.Precision.F64_STRICT:
+        name = {nb.IntegratorMethod.EULER: "euler",
+                nb.IntegratorMethod.VELOCITY_VERLET: "velocity-verlet",
+                nb.IntegratorMethod.RUNGE_KUTTA: "runge-kutta"}[integ]
+        ref = oracle.integrate(name, *b.arrays(), g, eps, dt, k)
+        for q, f in enumerate(("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z")):
+            assert np.array_equal(getattr(out[1], f), ref[q]), (integ.name, f)
+
+
+def test_integrate_stage_rejects_bad_arguments(gpu):
+    lib = nb.load_library()
+    db = nb.DeviceBodies(gpu, nb.plummer(64, 1))
+    assert lib.nbody_integrate_stage(gpu.handle, db.handle, 7, 1e-3) == nb.E_INVALID
+    assert lib.nbody_integrate_stage(gpu.handle, db.handle, -1, 1e-3) == nb.E_INVALID
+    assert lib.nbody_integrate_stage(gpu.handle, db.handle, 0, float("nan")) == nb.E_INVALID
+    db.close()
