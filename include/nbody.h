@@ -110,6 +110,18 @@ int nbody_ctx_kernel_times(nbody_ctx *ctx, double out[4]);
  * objects created afterwards. */
 int nbody_tuning_set_split(const int *parts, int count);
 int nbody_tuning_get_split(int *parts, int count);
+/* The force-kernel variant per bucket for one precision (F64, F32C or
+ * F32R; F64_STRICT has one kernel): variant = 3 * w + shape, w picking 1, 2
+ * or 4 waves per workgroup and shape one of three loop shapes (sources per
+ * chunk and chunks per compensated group in F64 and F32C, sources per LDS
+ * batch in F32R).  Loaded with the split table ("variant <precision>
+ * <bucket> <variant>" lines); default 0 (one wave per workgroup, shape 0).
+ * The shape sets the summation order of F64 and F32C (source parts are
+ * aligned to the chunk size; F32C also adds its compensated groups of
+ * chunks), so, like the split, a system keeps the variant of its size from
+ * its creation on. */
+int nbody_tuning_set_variant(int precision, const int *variants, int count);
+int nbody_tuning_get_variant(int precision, int *variants, int count);
 
 /* ---- device bodies: replaces DeviceBodies (src/gpu/device_bodies.rs:7-55).
  * create = DeviceBodies::new (:23-36, upload of the 7 SoA arrays plus zeroed

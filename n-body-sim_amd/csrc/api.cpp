@@ -307,6 +307,26 @@ int nbody_tuning_get_split(int* parts, int count) {
   });
 }
 
+int nbody_tuning_set_variant(int precision, const int* variants, int count) {
+  return guarded([&] {
+    require(variants && count > 0, NBODY_E_INVALID, "bad tuning table");
+    const int mode = mode_of(precision);
+    require(mode != nbody::MODE_F64_STRICT, NBODY_E_INVALID,
+            "NBODY_PRECISION_F64_STRICT has one kernel");
+    for (int b = 0; b < count; ++b)
+      require(variants[b] >= 0 && variants[b] < nbody::kForceVariants,
+              NBODY_E_INVALID, "variant out of range");
+    nbody::tuning_set_variant(mode, variants, count);
+  });
+}
+
+int nbody_tuning_get_variant(int precision, int* variants, int count) {
+  return guarded([&] {
+    require(variants && count > 0, NBODY_E_INVALID, "bad tuning table");
+    nbody::tuning_get_variant(mode_of(precision), variants, count);
+  });
+}
+
 // ---- device bodies ------------------------------------------------------
 int nbody_bodies_create(nbody_ctx* ctx, int64_t n, const double* masses,
                         const double* pos_x, const double* pos_y,

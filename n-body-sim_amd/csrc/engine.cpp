@@ -285,6 +285,7 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
   s.off = off;
   s.span = need;
   s.split = std::min(split_for_size(p.n), split_cap_);
+  for (int m = 0; m < 4; ++m) s.variant[m] = tuning_variant(m, p.n);
   // centre of mass and its velocity, f64 in index order (the centre of the
   // MODE_F32R ordering box, advanced by vcom * elapsed time), and the
   // largest coordinate distance from it (the box size)
@@ -464,6 +465,11 @@ int Engine::split_of(int id) const {
   return mode_ == MODE_F64_STRICT ? 1 : sys_[id].split;
 }
 
+// The force-kernel variant of a system in the current mode: chosen by its
+// own size when it was uploaded, so (like the split) it never depends on
+// what shares its launches.
+int Engine::variant_of(int id) const { return sys_[id].variant[mode_]; }
+
 // The split a system gets for its lifetime, fixed when it is uploaded so the
 // host work lists and the device descriptor can never disagree.
 int split_for_size(int64_t n) {
@@ -528,7 +534,14 @@ void Engine::run_round(std::vector<WorkItem>& items,
                            hipMemcpyHostToDevice, stream_),
             "hipMemcpy(work list)");
   const int nf = (int)items.size(), nc = (int)combine.size();
-  force_launch(d_items_, nf, potential);
+  // one force launch per kernel variant (items are grouped by variant)
+  for (int a = 0; a < nf;) {
+    const int v = variant_of(items[a].sys);
+    int b = a;
+    while (b < nf && variant_of(items[b].sys) == v) ++b;
+    force_launch(d_items_ + a, b - a, v, potential);
+    a = b;
+  }
   combine_launch(d_items_ + nf, nc);
   // the pageable host list must not change before the copy is consumed
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
@@ -592,10 +605,10 @@ void Engine::prepare(const std::vector<int>& ids) {
 // ones in the first), longest systems first.  flip: the snapshot parity of
 // the second of two consecutive steps.
 void Engine::step_spans(const std::vector<int>& order, int flip,
-                        std::vector<ItemSpan> (&force)[4],
+                        std::vector<ItemSpan> (&force)[4][kForceVariants],
                         std::vector<ItemSpan> (&combine)[4]) const {
   for (int r = 0; r < 4; ++r) {
-    force[r].clear();
+    for (auto& f : force[r]) f.clear();
     combine[r].clear();
     for (int id : order) {
       const Sys& s = sys_[id];
@@ -611,8 +624,8 @@ void Engine::step_spans(const std::vector<int>& order, int flip,
       }
       const int32_t tiles = (int32_t)((s.p.n + kWave - 1) / kWave);
       const int parts = split_of(id);
-      force[r].push_back(ItemSpan{0, id, tiles, (int16_t)stage, (int8_t)src,
-                                  (int8_t)parts});
+      force[r][variant_of(id)].push_back(
+          ItemSpan{0, id, tiles, (int16_t)stage, (int8_t)src, (int8_t)parts});
       if (parts > 1)
         combine[r].push_back(
             ItemSpan{0, id, tiles, (int16_t)stage, (int8_t)src, (int8_t)1});
@@ -629,9 +642,12 @@ std::vector<int> Engine::by_size(const std::vector<int>& ids) const {
 }
 
 void Engine::sort_items(std::vector<WorkItem>& items) const {
-  // longest systems first: the dispatcher then packs the short tail last
+  // grouped by kernel variant (one launch each), longest systems first
+  // within a group: the dispatcher then packs the short tail last
   std::stable_sort(items.begin(), items.end(),
                    [&](const WorkItem& a, const WorkItem& b) {
+                     const int va = variant_of(a.sys), vb = variant_of(b.sys);
+                     if (va != vb) return va < vb;
                      return sys_[a.sys].p.n > sys_[b.sys].p.n;
                    });
 }
@@ -685,16 +701,18 @@ void Engine::upload_step_lists(const std::vector<int>& ids, StepLists& L) {
       throw Error(NBODY_E_STATE, "system range outside the arena");
   }
   const std::vector<int> order = by_size(ids);
-  std::vector<ItemSpan> fa[4], ca[4], fb[4], cb[4];
+  std::vector<ItemSpan> fa[4][kForceVariants], ca[4], fb[4][kForceVariants], cb[4];
   step_spans(order, 0, fa, ca);
   step_spans(order, 1, fb, cb);
-  // one span array for the 16 lists; each list's items follow the previous
+  // one span array for every list; each list's items follow the previous
   std::vector<ItemSpan> spans;
   int64_t total = 0;
-  for (int k = 0; k < 16; ++k) {
-    const int which = k / 8, r = (k % 8) / 2, kind = k % 2;
+  for (int k = 0; k < StepLists::kLists; ++k) {
+    const int which = k / (4 * StepLists::kPer), r = (k / StepLists::kPer) % 4,
+              v = k % StepLists::kPer;
     const std::vector<ItemSpan>& l =
-        which == 0 ? (kind ? ca[r] : fa[r]) : (kind ? cb[r] : fb[r]);
+        v == kForceVariants ? (which == 0 ? ca[r] : cb[r])
+                            : (which == 0 ? fa[r][v] : fb[r][v]);
     L.off[k] = (int)total;
     for (ItemSpan sp : l) {
       sp.first = (int32_t)total;
@@ -738,12 +756,14 @@ void Engine::upload_step_lists(const std::vector<int>& ids, StepLists& L) {
   step_lists_ = L;
 }
 
-void Engine::force_launch(const WorkItem* items, int nitems, bool potential) {
+void Engine::force_launch(const WorkItem* items, int nitems, int variant,
+                          bool potential) {
   if (nitems <= 0) return;
   const bool timed = timer_ && timer_->on;
   if (timed) timer_->begin(stream_);
   hip_check(potential ? launch_potential(d_arena_, items, nitems, d_sys_, stream_)
-                      : launch_force(mode_, d_arena_, items, nitems, d_sys_, stream_),
+                      : launch_force(mode_, variant, d_arena_, items, nitems,
+                                     d_sys_, stream_),
             "force launch");
   if (timed) timer_->end(stream_, KernelTimer::FORCE);
 }
@@ -759,10 +779,12 @@ void Engine::combine_launch(const WorkItem* items, int nitems) {
 
 void Engine::launch_step(const StepLists& L, int which) {
   for (int r = 0; r < 4; ++r) {
-    const int k = which * 8 + r * 2;
-    if (L.cnt[k] == 0) continue;
-    force_launch(d_step_items_ + L.off[k], L.cnt[k]);
-    combine_launch(d_step_items_ + L.off[k + 1], L.cnt[k + 1]);
+    for (int v = 0; v < kForceVariants; ++v) {
+      const int k = StepLists::at(which, r, v);
+      force_launch(d_step_items_ + L.off[k], L.cnt[k], v);
+    }
+    const int c = StepLists::at(which, r, kForceVariants);
+    combine_launch(d_step_items_ + L.off[c], L.cnt[c]);
   }
 }
 
@@ -809,7 +831,7 @@ void Engine::run_steps(const std::vector<int>& ids, int64_t nsteps,
     // device work-list buffer and counts, so they stay valid while the
     // composition (and therefore every count) is unchanged
     int64_t key = (int64_t)(uintptr_t)d_step_items_ * 31 + mode_;
-    for (int k = 0; k < 16; ++k) key = key * 1000003 + L.cnt[k];
+    for (int k = 0; k < StepLists::kLists; ++k) key = key * 1000003 + L.cnt[k];
     if (key != gkey_ || ids != gids_ || !gexec_[0] || !gexec_[1]) {
       drop_graphs();
       for (int which = 0; which < 2; ++which) {

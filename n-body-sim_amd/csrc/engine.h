@@ -209,6 +209,8 @@ class Engine {
     bool prepared = false;
     int64_t steps = 0;
     int split = 1;              // source-range parts (fixed at upload)
+    int variant[4] = {0, 0, 0, 0};  // force-kernel variant per Mode (fixed
+                                    // at upload, from the tuning table)
     double com[3] = {0, 0, 0};  // centre of mass at upload (sort box centre,
     double vcom[3] = {0, 0, 0};  // advanced with the centre-of-mass velocity)
     double ext = 0.0;           // largest coordinate distance from com
@@ -241,16 +243,23 @@ class Engine {
   void tiles_of(int id, int stage, int src, std::vector<WorkItem>& out,
                 std::vector<WorkItem>* combine = nullptr) const;
   int split_of(int id) const;
-  struct StepLists {  // device offsets/counts: which*8 + round*2 + kind
-    int off[16] = {};
-    int cnt[16] = {};
+  int variant_of(int id) const;
+  // Device offsets/counts of one step's lists: for each step parity
+  // (which) and force round, one force list per kernel variant, then the
+  // combine list (v == kForceVariants).
+  struct StepLists {
+    static constexpr int kPer = kForceVariants + 1;
+    static constexpr int kLists = 2 * 4 * kPer;
+    static int at(int which, int round, int v) { return (which * 4 + round) * kPer + v; }
+    int off[kLists] = {};
+    int cnt[kLists] = {};
   };
   void run_round(std::vector<WorkItem>& items, std::vector<WorkItem>& combine,
                  bool potential = false);
   void sort_items(std::vector<WorkItem>& items) const;
   std::vector<int> by_size(const std::vector<int>& ids) const;
   void step_spans(const std::vector<int>& order, int flip,
-                  std::vector<ItemSpan> (&force)[4],
+                  std::vector<ItemSpan> (&force)[4][kForceVariants],
                   std::vector<ItemSpan> (&combine)[4]) const;
   void prime(const std::vector<int>& ids);
   void upload_step_lists(const std::vector<int>& ids, StepLists& L);
@@ -269,7 +278,8 @@ class Engine {
   void bh_step(const std::vector<int>& ids, int64_t t);
 
   // launch_force / launch_combine of nitems items, bracketed by the timer
-  void force_launch(const WorkItem* items, int nitems, bool potential = false);
+  void force_launch(const WorkItem* items, int nitems, int variant,
+                    bool potential = false);
   void combine_launch(const WorkItem* items, int nitems);
 
   hipStream_t stream_;

@@ -7,9 +7,18 @@
 namespace nbody {
 
 // One force evaluation over a work list, with each item's integrator stage
-// fused into the epilogue.  mode: Mode.  A, items, sys: device pointers.
-hipError_t launch_force(int mode, const Arena* A, const WorkItem* items,
-                        int nitems, const SysDesc* sys, hipStream_t s);
+// fused into the epilogue.  mode: Mode; variant: as above (MODE_F64_STRICT
+// has one kernel and ignores it).  A, items, sys: device pointers.
+hipError_t launch_force(int mode, int variant, const Arena* A,
+                        const WorkItem* items, int nitems, const SysDesc* sys,
+                        hipStream_t s);
+// The per-mode variant launchers (force_variants.hip, one object per mode).
+hipError_t launch_force_f64(int variant, const Arena* A, const WorkItem* items,
+                            int nitems, const SysDesc* sys, hipStream_t s);
+hipError_t launch_force_f32c(int variant, const Arena* A, const WorkItem* items,
+                             int nitems, const SysDesc* sys, hipStream_t s);
+hipError_t launch_force_f32r(int variant, const Arena* A, const WorkItem* items,
+                             int nitems, const SysDesc* sys, hipStream_t s);
 // Sums the partial results of split source ranges in part order, then runs
 // each target's stage epilogue (or stores its potential for STAGE_POT).
 hipError_t launch_combine(int mode, const Arena* A, const WorkItem* items,

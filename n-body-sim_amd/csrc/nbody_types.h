@@ -32,6 +32,11 @@ constexpr int kWave = 64;        // gfx950 wavefront
 constexpr int kBlock = NBODY_BLOCK;
 constexpr int kPad = 128;        // slack elements after the arena
 constexpr int kMaxSplit = 32;    // source-range parts per target tile
+// Force-kernel variants of the tuned modes (MODE_F64, MODE_F32C, MODE_F32R):
+// variant = 3 * w + shape, w indexing 1, 2 or 4 waves per workgroup (one
+// work item per wave) and shape the loop shape of force.h (chunk and group
+// sizes, LDS batch).  Variant 0 is one wave per workgroup, shape 0.
+constexpr int kForceVariants = 9;
 
 enum Mode : int32_t {            // force arithmetic (public NBODY_PRECISION_*)
   MODE_F64_STRICT = 0,           // reference operation order, no contraction,

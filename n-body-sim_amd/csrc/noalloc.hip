@@ -23,6 +23,7 @@
 #include "kernels.h"
 #include "noalloc.h"
 #include "nbody_types.h"
+#include "tuning.h"
 
 namespace nbody {
 
@@ -122,7 +123,8 @@ void noalloc_accelerations(int device, hipStream_t stream, int mode,
     base = (device >= 0 && device < kMaxDevices) ? g_ptr[device] : nullptr;
   }
   if (!base) throw Error(NBODY_E_STATE, "scratch not set up on this device");
-  // the same split as the single-system path, so the sums are the same bits
+  // the same split (and below, kernel variant) as the single-system path,
+  // so the sums are the same bits
   const int nsplit = mode == MODE_F64_STRICT ? 1 : split_for_size(n);
   const Layout L = layout(n, nsplit);
   if (L.end > kNoallocScratchBytes)
@@ -177,7 +179,9 @@ void noalloc_accelerations(int device, hipStream_t stream, int mode,
             "expand launch");
   hip_check(launch_prepare(d_arena, d_items, tiles, d_sys, stream),
             "prepare launch");
-  hip_check(launch_force(mode, d_arena, d_items + tiles, nforce, d_sys, stream),
+  // the variant the single-system path gives a system of n bodies
+  hip_check(launch_force(mode, tuning_variant(mode, n), d_arena, d_items + tiles,
+                         nforce, d_sys, stream),
             "force launch");
   hip_check(launch_combine(mode, d_arena, d_items + tiles + nforce, ncombine,
                            d_sys, stream),

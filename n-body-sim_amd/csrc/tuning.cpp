@@ -25,6 +25,7 @@ namespace {
 
 std::once_flag g_once;
 int g_split[kTuningBuckets];
+int g_variant[4][kTuningBuckets];  // by Mode; zero (the default kernel) at start
 
 int default_split(int b) {
   const int64_t n = (int64_t)1 << b;
@@ -77,26 +78,54 @@ void tuning_get(int* parts, int count) {
   for (int b = 0; b < kTuningBuckets && b < count; ++b) parts[b] = g_split[b];
 }
 
-// File format: lines "split <bucket> <parts>"; '#' starts a comment.
+int tuning_variant(int mode, int64_t n) {
+  std::call_once(g_once, init);
+  if (mode <= MODE_F64_STRICT || mode > MODE_F32R) return 0;
+  return g_variant[mode][bucket_of(std::max<int64_t>(n, 1))];
+}
+
+void tuning_set_variant(int mode, const int* v, int count) {
+  std::call_once(g_once, init);
+  if (mode <= MODE_F64_STRICT || mode > MODE_F32R) return;
+  for (int b = 0; b < kTuningBuckets && b < count; ++b)
+    g_variant[mode][b] = std::max(0, std::min(v[b], kForceVariants - 1));
+}
+
+void tuning_get_variant(int mode, int* v, int count) {
+  std::call_once(g_once, init);
+  for (int b = 0; b < kTuningBuckets && b < count; ++b)
+    v[b] = (mode <= MODE_F64_STRICT || mode > MODE_F32R) ? 0 : g_variant[mode][b];
+}
+
+// File format: lines "split <bucket> <parts>" and "variant <mode> <bucket>
+// <variant>" (mode: 1 f64, 2 f32c, 3 f32r); '#' starts a comment.
 bool tuning_load(const char* path) {
   FILE* f = std::fopen(path, "r");
   if (!f) return false;
   char line[256];
   int loaded = 0;
   int tmp[kTuningBuckets];
+  int tv[4][kTuningBuckets];
   std::memcpy(tmp, g_split, sizeof(tmp));
+  std::memcpy(tv, g_variant, sizeof(tv));
   while (std::fgets(line, sizeof(line), f)) {
     if (line[0] == '#') continue;
-    int b = -1, k = 0;
+    int b = -1, k = 0, m = -1;
     if (std::sscanf(line, "split %d %d", &b, &k) == 2 && b >= 0 &&
         b < kTuningBuckets && k >= 1) {
       tmp[b] = std::min(k, kMaxSplit);
+      ++loaded;
+    } else if (std::sscanf(line, "variant %d %d %d", &m, &b, &k) == 3 &&
+               m > MODE_F64_STRICT && m <= MODE_F32R && b >= 0 &&
+               b < kTuningBuckets && k >= 0 && k < kForceVariants) {
+      tv[m][b] = k;
       ++loaded;
     }
   }
   std::fclose(f);
   if (loaded == 0) return false;
   std::memcpy(g_split, tmp, sizeof(tmp));
+  std::memcpy(g_variant, tv, sizeof(tv));
   return true;
 }
 

@@ -65,6 +65,8 @@ def _bind(lib):
         "nbody_ctx_kernel_times": ([vp, _D], i),
         "nbody_tuning_set_split": ([ctypes.POINTER(i), i], i),
         "nbody_tuning_get_split": ([ctypes.POINTER(i), i], i),
+        "nbody_tuning_set_variant": ([i, ctypes.POINTER(i), i], i),
+        "nbody_tuning_get_variant": ([i, ctypes.POINTER(i), i], i),
         "nbody_bodies_create": ([vp, i64] + [_D] * 7 + [pp], i),
         "nbody_bodies_download": ([vp, vp] + [_D] * 7, i),
         "nbody_bodies_destroy": ([vp], i),
@@ -610,6 +612,20 @@ def tuning_split() -> list:
     arr = (ctypes.c_int * TUNING_BUCKETS)()
     _check(load_library().nbody_tuning_get_split(arr, TUNING_BUCKETS))
     return list(arr)
+
+
+def tuning_variant(precision: "Precision") -> list:
+    """Force-kernel variant per size bucket for one precision (0..8)."""
+    arr = (ctypes.c_int * 32)()
+    _check(load_library().nbody_tuning_get_variant(int(precision), arr, 32))
+    return list(arr)
+
+
+def set_tuning_variant(precision: "Precision", variants) -> None:
+    """Replaces one precision's variant table (systems created afterwards)."""
+    v = list(variants) + [0] * (32 - len(variants))
+    arr = (ctypes.c_int * 32)(*v[:32])
+    _check(load_library().nbody_tuning_set_variant(int(precision), arr, 32))
 
 
 def set_tuning_split(parts) -> None:

@@ -606,3 +606,69 @@ def test_integrate_stage_rejects_bad_arguments(gpu):
     assert lib.nbody_integrate_stage(gpu.handle, db.handle, -1, 1e-3) == nb.E_INVALID
     assert lib.nbody_integrate_stage(gpu.handle, db.handle, 0, float("nan")) == nb.E_INVALID
     db.close()
+
+
+def _with_variant_tables(tables, fn):
+    """Runs fn with the given variant tables ({precision: [32 entries]}),
+    then restores the loaded ones."""
+    saved = {p: nb.tuning_variant(p) for p in tables}
+    try:
+        for p, t in tables.items():
+            nb.set_tuning_variant(p, t)
+        return fn()
+    finally:
+        for p, t in saved.items():
+            nb.set_tuning_variant(p, t)
+
+
+@pytest.mark.parametrize("prec", [nb.Precision.F64, nb.Precision.F32C,
+                                  nb.Precision.F32R])
+def test_every_force_variant(gpu, prec):
+    """All nine kernel variants (1, 2 or 4 waves per workgroup x three loop
+    shapes) of each tuned mode meet the mode's tolerance against the oracle.
+    The shape sets the summation order of F64 and F32C (source parts are
+    aligned to the chunk size, and F32C's compensated groups are chunks), so
+    they give the same bits in every variant of one shape; F32R's chunks are
+    always 64 sources, so it gives variant 0's bits in every variant."""
+    b = nb.plummer(3000, 21)
+    ref = oracle.newton_parallel(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, 0.01)
+    tol = 1e-12 if prec == nb.Precision.F64 else 1e-5
+    got = []
+    for v in range(9):
+        out = _with_variant_tables({prec: [v] * 32},
+                                   lambda: gpu_acc(gpu, b, 1.0, 0.01, prec))
+        assert rel_err(out, ref).max() <= tol, (v, rel_err(out, ref).max())
+        got.append(out)
+    for v in range(9):
+        same = 0 if prec == nb.Precision.F32R else v % 3
+        for c in range(3):
+            assert np.array_equal(got[v][c], got[same][c]), (prec.name, v, c)
+
+
+def test_mixed_variants_alone_equals_batch(gpu):
+    """Systems of different size buckets tuned to different variants share
+    one batch (one launch per variant and round): each system's result is the
+    same alone and in the batch, bit for bit, in F32C (whose bits depend on
+    the shape: it is fixed per system from its own size)."""
+    table = [0] * 32
+    for b, v in zip(range(6, 14), (4, 8, 1, 5, 2, 7, 3, 6)):
+        table[b] = v
+    ns = (70, 150, 300, 700, 1300, 2500, 5000, 100)
+    systems = [nb.star_system(n, 900 + n) for n in ns]
+    integ = [k % 3 for k in range(len(ns))]
+
+    def run():
+        bt = nb.Batch(gpu, systems, integ, 1e-3, 1.0, 1e-6, nb.Precision.F32C)
+        bt.step(12)
+        mixed = bt.download()
+        offs = bt.offsets
+        bt.close()
+        for k, s in enumerate(systems):
+            one = nb.Batch(gpu, [s], [integ[k]], 1e-3, 1.0, 1e-6, nb.Precision.F32C)
+            one.step(12)
+            alone = one.download()
+            one.close()
+            for a, m in zip(alone, mixed):
+                assert np.array_equal(a, m[offs[k]:offs[k] + len(s)]), (k, len(s))
+
+    _with_variant_tables({nb.Precision.F32C: table}, run)
