@@ -771,7 +771,7 @@ def main():
     ap.add_argument("--f32", choices=sorted(F32_FORMS), default=None,
                     help="f32 form of the single and large headlines (default f32r; "
                          "the other form is timed beside it)")
-    ap.add_argument("--f32-batched", choices=sorted(F32_FORMS), default="f32r")
+    ap.add_argument("--f32-batched", choices=sorted(F32_FORMS), default="f32c")
     ap.add_argument("--f32-serve", choices=sorted(F32_FORMS), default="f32r")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)

@@ -19,7 +19,10 @@ chunk and chunks per compensated group in f64 and f32c, sources per LDS
 batch in f32r) is timed per mode and bucket, twice in opposite orders (the
 faster time counts).  Buckets below 2^14 bodies are timed as a batch of
 equal systems (the batched and served regime, graph-stepped velocity
-Verlet); larger ones as one system.  The fastest variant wins per mode.
+Verlet); larger ones as one system.  Systems of different variants in one
+batch need one launch per variant, so each mode takes one uniform variant
+(best summed over the buckets) and a bucket keeps its own best only where
+that is at least 8 % faster (choose_variants).
 
 The table is written as `split <bucket> <parts>` and `variant <mode>
 <bucket> <v>` lines (the format the library loads at start-up from
@@ -130,6 +133,27 @@ def time_variant(gpu, bodies, prec, budget_ms=150.0):
         bt.close()
 
 
+OVERRIDE_GAIN = 0.08      # a bucket leaves the uniform variant only if 8 % faster
+
+
+def choose_variants(times):
+    """Variant per bucket from times[b][v] (ms per step).  Systems of
+    different variants in one batch run in separate launches (one per
+    variant and round), so each launch drains before the next and the
+    longest-first order of the round is broken.  Hence one uniform variant
+    per mode (the smallest sum of times normalised by each bucket's best),
+    and a bucket takes its own best variant only where that is at least
+    OVERRIDE_GAIN faster than the uniform one.  Returns (uniform, {b: v})."""
+    nv = len(next(iter(times.values())))
+    score = [sum(t[v] / min(t) for t in times.values()) for v in range(nv)]
+    uniform = min(range(nv), key=lambda v: score[v])
+    pick = {}
+    for b, t in times.items():
+        best = min(range(nv), key=lambda v: t[v])
+        pick[b] = best if t[uniform] > (1.0 + OVERRIDE_GAIN) * t[best] else uniform
+    return uniform, pick
+
+
 def tune_variants(max_bucket=20, min_bucket=6, log=print):
     gpu = nb.GpuContext(0)
     report = []
@@ -140,6 +164,7 @@ def tune_variants(max_bucket=20, min_bucket=6, log=print):
         time_case(gpu, warm, nb.Precision.F64, budget_ms=200.0)
     for prec in VARIANT_MODES:
         table = nb.tuning_variant(prec)
+        measured = {}
         for b in range(min_bucket, max_bucket + 1):
             bodies = nb.plummer(1 << b, 11 + b)
             cands = list(range(9))
@@ -151,12 +176,16 @@ def tune_variants(max_bucket=20, min_bucket=6, log=print):
                     nb.set_tuning_variant(prec, trial)
                     t = time_variant(gpu, bodies, prec)
                     times[v] = min(times.get(v, t), t)
-            best = min(cands, key=lambda v: times[v])
-            table[b] = best
             nb.set_tuning_variant(prec, table)
+            measured[b] = [times[v] for v in cands]
             log(f"{prec.name} bucket {b:2d}: " +
-                " ".join(f"v{v}:{times[v]:.4f}" for v in cands) + f" -> {best}")
-            report.append((prec, b, [(v, times[v]) for v in cands], best))
+                " ".join(f"v{v}:{times[v]:.4f}" for v in cands))
+        uniform, pick = choose_variants(measured)
+        for b in range(min_bucket, max_bucket + 1):
+            table[b] = pick[b]
+            report.append((prec, b, list(enumerate(measured[b])), pick[b]))
+        log(f"{prec.name}: uniform variant {uniform}; per bucket {pick}")
+        nb.set_tuning_variant(prec, table)
         tables[prec] = table
     gpu.close()
     return tables, report
@@ -180,9 +209,16 @@ def read_comments(path, prefix):
     return out
 
 
+def current_table_path():
+    """The table the library loaded (NBODY_AMD_TUNING, else the one next to
+    the library)."""
+    return os.environ.get("NBODY_AMD_TUNING") or os.path.join(PKG, "tuning", "gfx950.txt")
+
+
 def write_table(path, split, split_report, variants, variant_report):
     day = time.strftime("%Y-%m-%d")
     lines = []
+    current = current_table_path()
     if split_report is not None:
         lines += ["# source-range parts per target tile by bucket floor(log2 n)",
                   "#   written by nbody_amd/autotune.py on " + day,
@@ -192,7 +228,7 @@ def write_table(path, split, split_report, variants, variant_report):
             lines.append("#   bucket %d n=%s: %s -> %d" % (
                 b, sizes, " ".join(f"{k}:{t:.3f}" for k, t in row), best))
     else:
-        lines += read_comments(path, "source-range parts")
+        lines += read_comments(current, "source-range parts")
     if variant_report is not None:
         lines += ["# force-kernel variant (3 * w + shape) by precision and bucket",
                   "#   written by nbody_amd/autotune.py on " + day,
@@ -202,7 +238,7 @@ def write_table(path, split, split_report, variants, variant_report):
             lines.append("#   %s bucket %d: %s -> %d" % (
                 prec.name, b, " ".join(f"{v}:{t:.4f}" for v, t in row), best))
     else:
-        lines += read_comments(path, "force-kernel variant")
+        lines += read_comments(current, "force-kernel variant")
     for b in range(len(split)):
         lines.append(f"split {b} {split[b]}")
     for prec in VARIANT_MODES:

@@ -212,3 +212,37 @@ def test_committed_tuning_table_parses():
     rows = [ln.split() for ln in open(path) if ln.startswith("split ")]
     assert len(rows) == 32
     assert all(1 <= int(k) <= 32 for _, _, k in rows)
+
+
+def test_tuning_variant_lines_and_abi(tmp_path):
+    """`variant <mode> <bucket> <v>` lines load per precision (1 f64, 2 f32c,
+    3 f32r); out-of-range entries are ignored; strict f64 has one kernel; the
+    set/get calls round-trip and reject out-of-range variants."""
+    import subprocess
+    import sys
+    f = tmp_path / "t.txt"
+    f.write_text("split 10 3\nvariant 2 10 7\nvariant 3 11 4\nvariant 1 12 9\n"
+                 "variant 0 10 5\nvariant 2 40 1\n")
+    code = ("import sys; sys.path.insert(0, %r); import nbody_amd as nb; P = nb.Precision; "
+            "print(nb.tuning_variant(P.F32C)[10], nb.tuning_variant(P.F32R)[11], "
+            "nb.tuning_variant(P.F64)[12], nb.tuning_variant(P.F64_STRICT)[10]); "
+            "t = nb.tuning_variant(P.F64); t[8] = 6; nb.set_tuning_variant(P.F64, t); "
+            "print(nb.tuning_variant(P.F64)[8]); "
+            "lib = nb.load_library(); import ctypes; a = (ctypes.c_int * 32)(*([9] * 32)); "
+            "print(lib.nbody_tuning_set_variant(int(P.F64), a, 32), "
+            "lib.nbody_tuning_set_variant(int(P.F64_STRICT), (ctypes.c_int * 32)(), 32))"
+            % nb._PKG_DIR)
+    env = dict(__import__("os").environ, NBODY_AMD_TUNING=str(f))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
+                         text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split() == ["7", "4", "0", "0", "6", str(nb.E_INVALID),
+                                  str(nb.E_INVALID)]
+
+
+def test_committed_tuning_variants_parse():
+    import os
+    path = os.path.join(nb._PKG_DIR, "tuning", "gfx950.txt")
+    rows = [ln.split() for ln in open(path) if ln.startswith("variant ")]
+    assert rows and all(1 <= int(m) <= 3 and 0 <= int(b) < 32 and 0 <= int(v) < 9
+                        for _, m, b, v in rows)
