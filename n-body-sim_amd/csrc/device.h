@@ -68,10 +68,11 @@ __device__ __forceinline__ void put_snapshot(const CArena* A, int dst,
   A->sx[dst][i] = x;
   A->sy[dst][i] = y;
   A->sz[dst][i] = z;
-  if (MODE == MODE_F32R) {
-    A->hx[dst][slot] = x;
-    A->hy[dst][slot] = y;
-    A->hz[dst][slot] = z;
+  if (MODE == MODE_F32R) {  // one 24-byte piece of the body's slot record
+    HSrc* h = A->hs[dst] + slot;
+    h->x = x;
+    h->y = y;
+    h->z = z;
   }
 }
 

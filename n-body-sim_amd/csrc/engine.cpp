@@ -155,19 +155,12 @@ size_t Engine::carve(char* base) {
   for (int q = 0; q < 9; ++q) A_.rk[q] = D();
   for (int q = 0; q < 3; ++q) A_.xs[q] = D();
   A_.phi = D();
-  A_.px = D(kMaxSplit);
-  A_.py = D(kMaxSplit);
-  A_.pz = D(kMaxSplit);
+  A_.pp = (Part3*)D(3 * kMaxSplit);  // record k of part p at p * pstride + k
   A_.pstride = (int64_t)(dbytes / sizeof(double));
-  for (int b = 0; b < 2; ++b) {
-    A_.hx[b] = D();
-    A_.hy[b] = D();
-    A_.hz[b] = D();
-  }
+  for (int b = 0; b < 2; ++b) A_.hs[b] = (HSrc*)D(4);
   A_.perm = (int32_t*)F();
   A_.iperm = (int32_t*)F();
   A_.gm32 = F();
-  A_.hg = F();
   A_.cap = cap_;
   return used;
 }

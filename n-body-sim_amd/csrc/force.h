@@ -459,9 +459,8 @@ __device__ __forceinline__ void chunk_f32r(const float4* stage,
 // as zero mass at the origin.
 template <bool NOMASK, int BATCH>
 __device__ __forceinline__ void sweep_f32r(float4* stage, float4* res,
-                                           const double* X, const double* Y,
-                                           const double* Z, const float* G,
-                                           int64_t jlo, int64_t jhi,
+                                           const HSrc* H, int64_t jlo,
+                                           int64_t jhi,
                                            int64_t end, double ox, double oy,
                                            double oz, const TargetR& t,
                                            float eps2, Kahan3& acc) {
@@ -472,10 +471,11 @@ __device__ __forceinline__ void sweep_f32r(float4* stage, float4* res,
     double dx = 0.0, dy = 0.0, dz = 0.0;
     float g = 0.f;
     if (valid) {
-      dx = X[j] - ox;
-      dy = Y[j] - oy;
-      dz = Z[j] - oz;
-      g = G[j];
+      const HSrc h = H[j];
+      dx = h.x - ox;
+      dy = h.y - oy;
+      dz = h.z - oz;
+      g = h.g;
     }
     const float sx = (float)dx, sy = (float)dy, sz = (float)dz;
     stage[lane] = {sx, sy, sz, g};
@@ -521,8 +521,8 @@ __global__ __launch_bounds__(kWave * W) void k_force(const Arena* Ap,
   if (MODE == MODE_F32R) {
     __shared__ float4 stage[W][2][kWave];  // staged s, residuals
     const int64_t ts = active ? slot : end - 1;  // in this tile
-    const double px0 = A->hx[src][ts], py0 = A->hy[src][ts],
-                 pz0 = A->hz[src][ts];
+    const HSrc h0 = A->hs[src][ts];
+    const double px0 = h0.x, py0 = h0.y, pz0 = h0.z;
     const double ox = lane_f64(px0, 32);  // the middle target
     const double oy = lane_f64(py0, 32);
     const double oz = lane_f64(pz0, 32);
@@ -548,13 +548,11 @@ __global__ __launch_bounds__(kWave * W) void k_force(const Arena* Ap,
     const float e2 = sd->eps2f;
     constexpr int B = Shape<S>::B;
     if (sd->flags & SYS_F32_SELF_FINITE)
-      sweep_f32r<true, B>(stage[wave][0], stage[wave][1], A->hx[src],
-                          A->hy[src], A->hz[src], A->hg, jlo, jhi, end, ox, oy,
-                          oz, t, e2, acc);
+      sweep_f32r<true, B>(stage[wave][0], stage[wave][1], A->hs[src], jlo,
+                          jhi, end, ox, oy, oz, t, e2, acc);
     else
-      sweep_f32r<false, B>(stage[wave][0], stage[wave][1], A->hx[src],
-                           A->hy[src], A->hz[src], A->hg, jlo, jhi, end, ox,
-                           oy, oz, t, e2, acc);
+      sweep_f32r<false, B>(stage[wave][0], stage[wave][1], A->hs[src], jlo,
+                           jhi, end, ox, oy, oz, t, e2, acc);
     ax = (double)acc.sx - (double)acc.cx;
     ay = (double)acc.sy - (double)acc.cy;
     az = (double)acc.sz - (double)acc.cz;
@@ -607,12 +605,11 @@ __global__ __launch_bounds__(kWave * W) void k_force(const Arena* Ap,
   if (sd->nsplit > 1) {  // partial sum; k_combine finishes the target
     // partials in body order in every mode (in MODE_F32R a wave's 64 writes
     // are scattered, so k_combine can walk bodies with coalesced accesses)
-    const int64_t q = (int64_t)it.part * A->pstride + i;
-    A->px[q] = ax;
-    if (!POT) {
-      A->py[q] = ay;
-      A->pz[q] = az;
-    }
+    Part3* q = A->pp + ((int64_t)it.part * A->pstride + i);
+    if (POT)
+      q->x = ax;
+    else
+      *q = Part3{ax, ay, az};
     return;
   }
   if (POT) {

@@ -44,9 +44,9 @@ hipError_t launch_gather(const Arena* A, const WorkItem* items, int nitems,
 // MODE_F32R body order: keys (rank << 48 | Hilbert index) and local indices
 // of the listed systems' bodies, a radix sort (sort_pairs: hipCUB; query the
 // temporary size with tmp == nullptr), then perm from the sorted values.
-// launch_sort_scatter also rebuilds the slot-ordered snapshot copy (hx, hy,
-// hz, hg) from perm; with vals == nullptr it keeps perm and only rebuilds the
-// copy (after the body-order snapshot or g*m was rewritten).
+// launch_sort_scatter also rebuilds the slot-ordered snapshot copy (hs: the
+// positions and g*m) from perm; with vals == nullptr it keeps perm and only
+// rebuilds the copy (after the body-order snapshot or g*m was rewritten).
 hipError_t launch_sort_keys(const Arena* A, const SortSys* tab, int nsys,
                             int max_tiles, uint64_t* keys, int32_t* vals,
                             hipStream_t s);

@@ -40,17 +40,19 @@ __global__ __launch_bounds__(kBlock) void k_combine(const Arena* Ap,
                     (int64_t)(threadIdx.x & (kWave - 1));
   if (k >= end) return;
   const int64_t st = A->pstride;
-  double ax = A->px[k];
+  const Part3* pp = A->pp;
+  double ax = pp[k].x;
   if (it.stage == STAGE_POT) {
-    for (int p = 1; p < sd->nsplit; ++p) ax = ax + A->px[p * st + k];
+    for (int p = 1; p < sd->nsplit; ++p) ax = ax + pp[p * st + k].x;
     A->phi[k] = ax;
     return;
   }
-  double ay = A->py[k], az = A->pz[k];
+  double ay = pp[k].y, az = pp[k].z;
   for (int p = 1; p < sd->nsplit; ++p) {
-    ax = ax + A->px[p * st + k];
-    ay = ay + A->py[p * st + k];
-    az = az + A->pz[p * st + k];
+    const Part3 q = pp[p * st + k];
+    ax = ax + q.x;
+    ay = ay + q.y;
+    az = az + q.z;
   }
   // the body's slot in Hilbert order, for MODE_F32R's slot-ordered snapshot
   const int64_t slot = MODE == MODE_F32R ? sd->off + A->iperm[k] : k;
@@ -398,12 +400,9 @@ __global__ __launch_bounds__(kWave) void k_sort_scatter(const Arena* Ap,
   // the slot-ordered copy of both snapshot buffers (only the current one
   // matters; the next force round rewrites the other) and of g*m
   const int64_t i = t.off + v;
-  for (int b = 0; b < 2; ++b) {
-    A->hx[b][q] = A->sx[b][i];
-    A->hy[b][q] = A->sy[b][i];
-    A->hz[b][q] = A->sz[b][i];
-  }
-  A->hg[q] = A->gm32[i];
+  const float g = A->gm32[i];
+  for (int b = 0; b < 2; ++b)
+    A->hs[b][q] = HSrc{A->sx[b][i], A->sy[b][i], A->sz[b][i], g, 0.f};
 }
 
 }  // namespace

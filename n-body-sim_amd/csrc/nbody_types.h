@@ -6,14 +6,17 @@
 //   sx/sy/sz[2]                   f64 source snapshot, double-buffered
 //   perm, iperm                   MODE_F32R body order (Hilbert curve)
 //                                 and its inverse
-//   hx/hy/hz[2], hg               MODE_F32R: the snapshot and g*m in that
-//                                 order, so sources read coalesced
+//   hs[2]                         MODE_F32R: the snapshot and g*m in that
+//                                 order, one 32-byte record per slot, so
+//                                 sources read coalesced and a body's
+//                                 scattered update is one record
 //   ax, ay, az                    carried acceleration (velocity Verlet) or the
 //                                 output of a plain force evaluation
 //   rk[9]                         RK4 running sums and next-stage velocity
 //   xs[3]                         staged RK4: positions at the step's start
 //   phi                           per-body potential (energy diagnostics)
-//   px/py/pz[kMaxSplit]           partial sums of split source ranges
+//   pp[kMaxSplit]                 partial sums of split source ranges,
+//                                 one 24-byte record per body and part
 //                                 (body order in every mode)
 // Systems occupy disjoint [off, off+n) ranges of every array.  Every array has
 // kPad elements of zeroed slack past the arena end so a source chunk that
@@ -116,6 +119,21 @@ struct ItemSpan {
 };
 static_assert(sizeof(ItemSpan) == 16, "ItemSpan layout");
 
+// One body's partial acceleration sum over one source part (k_force ->
+// k_combine); in the potential pass only x is used.
+struct Part3 {
+  double x, y, z;
+};
+static_assert(sizeof(Part3) == 24, "Part3 layout");
+
+// One MODE_F32R source in slot order: its snapshot position and g*m in f32.
+struct HSrc {
+  double x, y, z;
+  float g;
+  float pad_;
+};
+static_assert(sizeof(HSrc) == 32, "HSrc layout");
+
 struct Arena {         // device pointers, passed by value to every kernel
   double *m, *gm64;
   float *gm32;
@@ -125,13 +143,12 @@ struct Arena {         // device pointers, passed by value to every kernel
   double *rk[9];
   double *xs[3];        // staged RK4: positions at the start of the step
   double *phi;
-  double *px, *py, *pz; // per-part partial sums: [part * pstride + i]
+  Part3* pp;            // per-part partial sums: [part * pstride + i]
   int32_t* perm;        // MODE_F32R order: slot off + k holds body
                         // off + perm[off + k] (Hilbert order)
   int32_t* iperm;       // its inverse: body off + v sits at slot
                         // off + iperm[off + v]
-  double *hx[2], *hy[2], *hz[2];  // MODE_F32R: sx/sy/sz in slot order
-  float* hg;            // MODE_F32R: gm32 in slot order
+  HSrc* hs[2];          // MODE_F32R: sx/sy/sz[b] and gm32 in slot order
   int64_t pstride;     // cap + kPad
   int64_t cap;         // bodies (excluding the kPad slack)
 };

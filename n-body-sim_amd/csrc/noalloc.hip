@@ -39,7 +39,7 @@ std::mutex g_mu;
 void* g_ptr[kMaxDevices] = {};  // scratch's address on each device
 
 struct Layout {  // byte offsets into the scratch array
-  size_t arena, sys, spans, items, gm64, gm32, sx, sy, sz, px, py, pz, end;
+  size_t arena, sys, spans, items, gm64, gm32, sx, sy, sz, pp, end;
   int64_t len;   // elements per per-body array (n + kPad, rounded)
 };
 
@@ -64,9 +64,7 @@ Layout layout(int64_t n, int nsplit) {
   L.sx = take((size_t)L.len * sizeof(double));
   L.sy = take((size_t)L.len * sizeof(double));
   L.sz = take((size_t)L.len * sizeof(double));
-  L.px = take((size_t)nsplit * L.len * sizeof(double));
-  L.py = take((size_t)nsplit * L.len * sizeof(double));
-  L.pz = take((size_t)nsplit * L.len * sizeof(double));
+  L.pp = take((size_t)nsplit * L.len * sizeof(Part3));
   L.end = o;
   return L;
 }
@@ -143,9 +141,7 @@ void noalloc_accelerations(int device, hipStream_t stream, int mode,
   a.ax = ax;
   a.ay = ay;
   a.az = az;
-  a.px = (double*)(s + L.px);
-  a.py = (double*)(s + L.py);
-  a.pz = (double*)(s + L.pz);
+  a.pp = (Part3*)(s + L.pp);
   a.pstride = L.len;
   a.cap = n;
   SysDesc d{};
