@@ -26,7 +26,12 @@ accuracy against the f64 oracle.
 A "step" of the headline is one velocity-Verlet step of the whole system;
 interactions are N(N-1) per force evaluation, 20 flops each.  Inputs are
 resident in HBM before timing starts; each timed region is bracketed by HIP
-events on the library's stream.  `roofline` is the force kernel's: its
+events on the library's stream.  Before each single-system form (and before
+any other workload run on its own) the GPU runs `--clock-warm` seconds (0.5)
+of untimed force evaluations of the same kind on a throwaway copy of the
+system: the clock drops while the host works between measurements and
+ramps up again over tens of milliseconds, and a 20-step timed region would
+otherwise average over that ramp.  `roofline` is the force kernel's: its
 launches inside the timed region are bracketed by HIP events, and achieved
 FLOP/s = 20 x interactions per launch / mean launch time.
 
@@ -189,13 +194,34 @@ def other_f32(form):
     return "f32c" if form == "f32r" else "f32r"
 
 
+def warm_clocks(gpu, seconds, prec=None, bodies=None):
+    """Untimed force evaluations for about `seconds` right before a timed
+    region, on a throwaway copy of the system (`bodies`, else a seeded
+    N=65536 sphere) in the precision about to be timed.  The GPU clock drops
+    while the host works between measurements (allocation, energies,
+    accuracy against the oracle) and ramps up again over tens of
+    milliseconds of sustained load; without this a short timed region would
+    average over that ramp rather than measure the sustained rate."""
+    if seconds <= 0:
+        return
+    grav = nb.GpuNewtonParallelGravity(1.0, 0.005 ** 2,
+                                       prec if prec is not None else nb.Precision.F32C)
+    db = nb.DeviceBodies(gpu, bodies if bodies is not None else nb.plummer(65536, 99))
+    t_end = time.perf_counter() + seconds
+    while time.perf_counter() < t_end:
+        grav.calculate_accelerations(gpu, db, download=False)
+        gpu.synchronize()
+    db.close()
+
+
 # ---- config 2: one N=65536 Plummer sphere, velocity Verlet -------------------
-def time_single(gpu, b, prec, g, eps, dt, steps, warmup):
+def time_single(gpu, b, prec, g, eps, dt, steps, warmup, clock_warm):
     grav = nb.GpuNewtonParallelGravity(g, eps * eps, prec)
     integ = nb.GpuVelocityVerletIntegrator(dt)
     db = nb.DeviceBodies(gpu, b)
     acc0 = grav.calculate_accelerations(gpu, db)
     e0 = energy(gpu, db, g, eps)
+    warm_clocks(gpu, clock_warm, prec, b)
     integ.step(gpu, db, grav, warmup)
     gpu.synchronize()
     gpu.kernel_timing(True)
@@ -222,7 +248,8 @@ def run_single(args, gpu):
     res = {}
     for name in forms:
         log(f"single {name}: N={n} VV {args.steps} steps (+{args.warmup} warmup)")
-        res[name] = time_single(gpu, b, PRECS[name], g, eps, dt, args.steps, args.warmup)
+        res[name] = time_single(gpu, b, PRECS[name], g, eps, dt, args.steps, args.warmup,
+                                args.clock_warm)
         log(f"  {res[name][0]:.1f} ms -> {inter / (res[name][0] / 1e3):.4e} int/s; "
             f"force kernel {res[name][1]['force_ms'] / max(res[name][1]['force_launches'], 1):.3f} ms")
     traffic = measured_traffic() or {}
@@ -771,9 +798,12 @@ def main():
     ap.add_argument("--f32", choices=sorted(F32_FORMS), default=None,
                     help="f32 form of the single and large headlines (default f32r; "
                          "the other form is timed beside it)")
-    ap.add_argument("--f32-batched", choices=sorted(F32_FORMS), default="f32c")
+    ap.add_argument("--f32-batched", choices=sorted(F32_FORMS), default="f32r")
     ap.add_argument("--f32-serve", choices=sorted(F32_FORMS), default="f32r")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--clock-warm", type=float, default=0.5,
+                    help="seconds of untimed force evaluations before the "
+                         "first workload (GPU clock ramp)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -789,6 +819,8 @@ def main():
     name, cus = gpu.device_info()
     log(f"device {name}, {cus} CUs")
     t_start = time.perf_counter()
+    if args.workload not in ("copy", "single", "all"):
+        warm_clocks(gpu, args.clock_warm)
     if args.workload == "all":
         rec = run_single(args, gpu)
         rec["batched"] = with_common(run_batched(args, gpu),
