@@ -59,10 +59,20 @@ extern "C" {
 #define NBODY_PRECISION_F32R 3        /* bodies in Hilbert order, coordinates relative to each wave's middle target split into an f32 value and an f32 residual; f32 pair terms, the residuals added for source chunks near the targets; compensated sums; <= 1e-5 relative; faster than F32C for large systems */
 /* Zero softening (eps2 == 0) with two distinct bodies at one point:
  * F64_STRICT, F64 and F32C follow the reference (1/sqrt(0) * 0 gives NaN
- * for both bodies).  F32R excludes a target's own term by dropping pairs at
- * r2 == 0, so such bodies get the finite sum over their other sources. */
-/* The f32 forms need pair distances below about 1e18 (r^2 in f32 range);
- * wider systems need NBODY_PRECISION_F64. */
+ * for both bodies).  F32R excludes a target's own term by its zero
+ * difference, so such bodies get the finite sum over their other sources. */
+/* Range of the f32 forms (F32C, F32R): r^2 and k = g*m*(r^2 + eps2)^-1.5
+ * are formed in f32.
+ *  - Pair distances must stay below about 1e18 (r^2 in f32 range); wider
+ *    systems need NBODY_PRECISION_F64.
+ *  - Tiny softening: once g*m*eps^-3 exceeds FLT_MAX (eps below about
+ *    1.4e-13 for g*m = 1), k overflows at close range.  F32C skips the
+ *    target's own index.  F32R drops its own pair by the zero difference:
+ *    it runs unmasked only while g*max(m)*eps^-3 < 2^120 (the self pair
+ *    then adds exactly 0 * k); otherwise it clamps k to +-FLT_MAX, so a pair
+ *    at zero difference adds zero and every other pair stays finite.
+ *  - Distinct bodies closer than the softening can also overflow k in F32C
+ *    when g*m*eps^-3 exceeds FLT_MAX; use NBODY_PRECISION_F64 there. */
 
 typedef struct nbody_ctx nbody_ctx;
 typedef struct nbody_bodies nbody_bodies;
