@@ -384,6 +384,69 @@ def test_large_workload_full_size_sampled(gpu):
         assert e.max() <= tol, prec.name
 
 
+@pytest.mark.parametrize("prec", [nb.Precision.F64_STRICT, nb.Precision.F32R])
+def test_large_workload_rk4_step_sampled(gpu, prec):
+    """One RK4 step of the N = 1,048,576 colliding spheres (softening 0.002,
+    dt 2.5e-4), run stage by stage through nbody_integrate_stage and checked
+    on 1024 sampled bodies:
+    - each stage's accelerations against the oracle's direct sum at the
+      positions that stage evaluated (all 1M sources): bit for bit in the
+      strict form, within 1e-5 in F32R;
+    - each stage's positions and the step's result against the per-body RK4
+      update of runge_kutta.rs:149-281 (numpy, no contraction) from the
+      step's start and the stage accelerations, bit for bit;
+    - the staged step against the fused nbody_step, bit for bit."""
+    n, eps, dt = 1 << 20, 0.002, 2.5e-4
+    b = nb.colliding(n, 5, separation=10.0, rel_speed=0.5)
+    idx = np.sort(np.random.default_rng(22).choice(n, 1024, replace=False))
+    threads = min(16, oracle.default_threads())
+    grav = nb.GpuNewtonParallelGravity(1.0, eps * eps, prec)
+    lib = nb.load_library()
+    x0 = [a[idx] for a in (b.pos_x, b.pos_y, b.pos_z)]
+    v0 = [a[idx] for a in (b.vel_x, b.vel_y, b.vel_z)]
+    h, s6 = dt / 2.0, dt / 6.0
+    db = nb.DeviceBodies(gpu, b)
+    acc, kr = [], [v0]
+    expect = x0
+    for s, stage in enumerate(nb.STAGE_RK4):
+        cur = db.download()
+        pos = (cur.pos_x, cur.pos_y, cur.pos_z)
+        for q in range(3):
+            assert np.array_equal(pos[q][idx], expect[q]), (s, q)
+        got = tuple(a[idx] for a in grav.calculate_accelerations(gpu, db))
+        ref = oracle.newton_targets(b.masses, *pos, 1.0, eps, idx, threads)
+        if prec == nb.Precision.F64_STRICT:
+            for q in range(3):
+                assert np.array_equal(got[q], ref[q]), (s, q)
+        else:
+            e = rel_err(got, ref)
+            print(f"stage {s + 1}: max {e.max():.3e} median {np.median(e):.3e}",
+                  flush=True)
+            assert e.max() <= 1e-5, s
+        acc.append(got)
+        nb._check(lib.nbody_integrate_stage(gpu.handle, db.handle, stage, dt))
+        if s < 3:
+            c = h if s < 2 else dt
+            kr.append([v0[q] + c * got[q] for q in range(3)])     # k_{s+2,r}
+            expect = [x0[q] + c * kr[s][q] for q in range(3)]      # next stage point
+    staged = db.download()
+    db.close()
+    for q, f in enumerate(("pos_x", "pos_y", "pos_z")):
+        want = x0[q] + s6 * (((kr[0][q] + (2.0 * kr[1][q])) + (2.0 * kr[2][q]))
+                             + kr[3][q])
+        assert np.array_equal(getattr(staged, f)[idx], want), f
+    for q, f in enumerate(("vel_x", "vel_y", "vel_z")):
+        want = v0[q] + s6 * (((acc[0][q] + (2.0 * acc[1][q])) + (2.0 * acc[2][q]))
+                             + acc[3][q])
+        assert np.array_equal(getattr(staged, f)[idx], want), f
+    db = nb.DeviceBodies(gpu, b)
+    nb.IntegratorMethod.RUNGE_KUTTA.gpu_create(dt).step(gpu, db, grav, 1)
+    fused = db.download()
+    db.close()
+    for f in ("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z"):
+        assert np.array_equal(getattr(fused, f), getattr(staged, f)), f
+
+
 def _batched_workload():
     """The batched workload at its full shape: 512 star systems, N
     log-uniform in [64, 8192], Euler / VV / RK4 in turn."""

@@ -92,6 +92,30 @@ def test_strict_server_matches_oracle(gpu):
         assert steps == sorted(steps)
 
 
+@pytest.mark.parametrize("precision", [nb.Precision.F32C, nb.Precision.F32R])
+def test_f32_server_matches_oracle(gpu, precision):
+    """Served results of the f32-compensated forms against the oracle's f64
+    trajectories of the same requests (every integrator, N 32..900, 10 to 60
+    steps): final positions within 1e-9 of the system's extent, final
+    velocities within 1e-7 of its largest speed (measured: up to 1.0e-11
+    and 7.2e-9)."""
+    reqs = requests(6, 2)
+    ids, done, _, _ = run_server(reqs, precision, snapshots=False)
+    for rid, r in zip(ids, reqs):
+        b = r["bodies"]
+        ref = oracle.integrate(int(r["integrator"]), *b.arrays(), 1.0, 0.01, r["dt"],
+                               r["steps"])
+        ev, data = done[rid]
+        assert ev["kind"] == nb.EVENT_DONE and ev["step"] == r["steps"]
+        extent = max(np.abs(ref[q]).max() for q in range(3))
+        speed = max(np.abs(ref[q]).max() for q in range(3, 6))
+        dpos = max(np.abs(data[q] - ref[q]).max() for q in range(3)) / extent
+        dvel = max(np.abs(data[q] - ref[q]).max() for q in range(3, 6)) / speed
+        print(f"{precision.name} n={len(b)} {r['integrator'].name} {r['steps']} steps: "
+              f"position {dpos:.2e} velocity {dvel:.2e}", flush=True)
+        assert dpos <= 1e-9 and dvel <= 1e-7, (len(b), dpos, dvel)
+
+
 def test_admission_by_memory_and_backlog(gpu):
     # arena of 1024 bodies: requests of 600 bodies cannot be resident together
     reqs = [dict(bodies=nb.plummer(600, 30 + k), integrator=nb.IntegratorMethod.EULER,
