@@ -22,7 +22,8 @@ equal systems (the batched and served regime, graph-stepped velocity
 Verlet); larger ones as one system.  Systems of different variants in one
 batch need one launch per variant, so each mode takes one uniform variant
 (best summed over the buckets) and a bucket keeps its own best only where
-that is at least 8 % faster (choose_variants).
+that is at least 8 % faster (choose_variants).  Buckets outside the timed
+range (below 2^6 and above the largest timed one) take the uniform variant.
 
 The table is written as `split <bucket> <parts>` and `variant <mode>
 <bucket> <v>` lines (the format the library loads at start-up from
@@ -181,6 +182,10 @@ def tune_variants(max_bucket=20, min_bucket=6, log=print):
             log(f"{prec.name} bucket {b:2d}: " +
                 " ".join(f"v{v}:{times[v]:.4f}" for v in cands))
         uniform, pick = choose_variants(measured)
+        # buckets outside the timed range take the uniform variant too, so a
+        # round mixing very small (or very large) systems with the rest
+        # stays one launch
+        table = [uniform] * len(table)
         for b in range(min_bucket, max_bucket + 1):
             table[b] = pick[b]
             report.append((prec, b, list(enumerate(measured[b])), pick[b]))
@@ -233,7 +238,10 @@ def write_table(path, split, split_report, variants, variant_report):
         lines += ["# force-kernel variant (3 * w + shape) by precision and bucket",
                   "#   written by nbody_amd/autotune.py on " + day,
                   "#   ms/step per variant (velocity Verlet, Plummer; buckets below "
-                  f"{BATCH_BELOW} as a batch):"]
+                  f"{BATCH_BELOW} as a batch);",
+                  "#   one uniform variant per precision, a bucket's own best where "
+                  f">= {round(100 * OVERRIDE_GAIN)} % faster;",
+                  "#   buckets outside the timed range take the uniform variant:"]
         for prec, b, row, best in variant_report:
             lines.append("#   %s bucket %d: %s -> %d" % (
                 prec.name, b, " ".join(f"{v}:{t:.4f}" for v, t in row), best))

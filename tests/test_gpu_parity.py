@@ -712,6 +712,15 @@ def test_every_force_variant(gpu, prec):
         same = 0 if prec == nb.Precision.F32R else v % 3
         for c in range(3):
             assert np.array_equal(got[v][c], got[same][c]), (prec.name, v, c)
+    # systems of one partial tile and ragged ends: a multi-wave workgroup
+    # with fewer work items than waves leaves its extra waves idle
+    for n in (2, 37, 64, 100, 130):
+        bs = nb.plummer(n, 60 + n)
+        refs = oracle.newton_parallel(bs.masses, bs.pos_x, bs.pos_y, bs.pos_z, 1.0, 0.01)
+        for v in range(9):
+            out = _with_variant_tables({prec: [v] * 32},
+                                       lambda: gpu_acc(gpu, bs, 1.0, 0.01, prec))
+            assert rel_err(out, refs).max() <= tol, (n, v, rel_err(out, refs).max())
 
 
 def test_mixed_variants_alone_equals_batch(gpu):
