@@ -54,6 +54,7 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "n-body-sim_amd")]
 import nbody_amd as nb  # noqa: E402
 
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}  # MI355X vector peaks (datasheet)
+HBM_PEAK_GBS = 8000.0                        # MI355X HBM3E (datasheet)
 FLOPS_PER_INTERACTION = 20
 # one GPU's share of a box's host cores: worker pools are capped at this
 HOST_CPU_SHARE = 16
@@ -177,6 +178,12 @@ def roofline(dtype, interactions, kt, step_rate=None, traffic=None):
          "combine_launches": kt["combine_launches"]}
     if step_rate is not None:
         r["frac_of_whole_step"] = step_rate * FLOPS_PER_INTERACTION / 1e12 / PEAK_TFLOPS[dtype]
+    if traffic:
+        # measured HBM bytes per launch over the live launch time, against
+        # MI355X's 8 TB/s
+        gbs = traffic / (r["force_kernel_avg_ms"] / 1e3) / 1e9
+        r["hbm_achieved_gbs"] = gbs
+        r["hbm_frac"] = gbs / HBM_PEAK_GBS
     return r
 
 
