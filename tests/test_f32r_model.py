@@ -1,4 +1,4 @@
-"""CPU model of MODE_F32R's near/far chunk rule (n-body-sim_amd/csrc/kernels.hip,
+"""CPU model of MODE_F32R's near/far chunk rule (n-body-sim_amd/csrc/force.h,
 near_source / sweep_f32r / chunk_f32r).
 
 Targets and sources run in Hilbert order; each wave of 64 targets takes its

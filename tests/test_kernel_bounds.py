@@ -1,8 +1,8 @@
 """CPU model of the force kernel's memory-access pattern (no GPU needed).
 
 Replays, for every (n, tile, chunk size), the source regions the kernel walks
-(kernels.hip: make_bounds and the sweep loops, including the prefetch of one
-chunk ahead) and checks that every source index stays inside the system's
+(force.h: make_bounds, part_range and the sweep loops, including the prefetch
+of one chunk ahead) and checks that every source index stays inside the system's
 arena span plus the arena's zeroed slack, that the regions cover the
 system's sources exactly once in index order, and that the masked regions
 are the only ones that can contain j == i or j >= n.
@@ -28,7 +28,7 @@ def bounds(off, n, t0, u):
 
 
 def clip_part(regions, off, n, u, nsplit, part):
-    """kernels.hip clip_part: restrict regions to one part's chunk run."""
+    """force.h part_range: restrict regions to one part's chunk run."""
     if nsplit <= 1:
         return regions
     nchunks = (n + u - 1) // u
@@ -83,7 +83,7 @@ def test_slack_covers_largest_prefetch():
     assert ALIGN % 8 == 0 and K_WAVE % 8 == 0
 
 
-@pytest.mark.parametrize("u", [4, 8])
+@pytest.mark.parametrize("u", [4, 8, 64])
 def test_parts_cover_sources_once_in_order(u):
     for n in (513, 1000, 1024, 4096, 8191, 65536 // 16):
         off = 2 * ALIGN
