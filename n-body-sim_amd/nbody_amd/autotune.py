@@ -155,6 +155,17 @@ def choose_variants(times):
     return uniform, pick
 
 
+def variant_table(uniform, pick, nbuckets):
+    """The full table of one mode: the chosen variant in the timed buckets
+    (pick: {bucket: v}), the uniform variant everywhere else, so a round
+    mixing very small (or very large) systems with the rest stays one
+    launch."""
+    table = [uniform] * nbuckets
+    for b, v in pick.items():
+        table[b] = v
+    return table
+
+
 def tune_variants(max_bucket=20, min_bucket=6, log=print):
     gpu = nb.GpuContext(0)
     report = []
@@ -182,12 +193,8 @@ def tune_variants(max_bucket=20, min_bucket=6, log=print):
             log(f"{prec.name} bucket {b:2d}: " +
                 " ".join(f"v{v}:{times[v]:.4f}" for v in cands))
         uniform, pick = choose_variants(measured)
-        # buckets outside the timed range take the uniform variant too, so a
-        # round mixing very small (or very large) systems with the rest
-        # stays one launch
-        table = [uniform] * len(table)
+        table = variant_table(uniform, pick, len(table))
         for b in range(min_bucket, max_bucket + 1):
-            table[b] = pick[b]
             report.append((prec, b, list(enumerate(measured[b])), pick[b]))
         log(f"{prec.name}: uniform variant {uniform}; per bucket {pick}")
         nb.set_tuning_variant(prec, table)
