@@ -809,8 +809,9 @@ def main():
     ap.add_argument("--f32-serve", choices=sorted(F32_FORMS), default="f32r")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--clock-warm", type=float, default=0.5,
-                    help="seconds of untimed force evaluations before the "
-                         "first workload (GPU clock ramp)")
+                    help="seconds of untimed force evaluations before each "
+                         "single-system form and before any other workload run "
+                         "on its own (GPU clock ramp)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1:
