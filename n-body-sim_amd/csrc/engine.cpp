@@ -307,9 +307,8 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
     hip_check(hipMemcpyAsync(dst[q] + off, arrs[q], bytes,
                              hipMemcpyHostToDevice, stream_),
               "hipMemcpy(upload)");
-  hip_check(hipMemsetAsync(A_.ax + off, 0, bytes, stream_), "hipMemset(acc)");
-  hip_check(hipMemsetAsync(A_.ay + off, 0, bytes, stream_), "hipMemset(acc)");
-  hip_check(hipMemsetAsync(A_.az + off, 0, bytes, stream_), "hipMemset(acc)");
+  hip_check(launch_clear_range(d_arena_, off, p.n, CLEAR_ACC, stream_),
+            "clear launch");
   // the caller's arrays may be released as soon as this returns
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   table_dirty_ = true;
@@ -322,12 +321,10 @@ void Engine::remove(int id) {
   check_id(id);
   trees_.erase(id);
   Sys& s = sys_[id];
-  // Clear the range so stale values can never be read as sources.
-  for (double* a : {A_.x, A_.y, A_.z, A_.sx[0], A_.sy[0], A_.sz[0], A_.sx[1],
-                    A_.sy[1], A_.sz[1], A_.gm64})
-    hip_check(launch_fill_zero(a + s.off, s.span, stream_), "fill");
-  hip_check(hipMemsetAsync(A_.gm32 + s.off, 0, sizeof(float) * s.span, stream_),
-            "hipMemset");
+  // Clear the range so stale values can never be read as sources (one
+  // launch for every array).
+  hip_check(launch_clear_range(d_arena_, s.off, s.span, CLEAR_RETIRED, stream_),
+            "clear launch");
   int64_t off = s.off, len = s.span;
   auto nx = free_.lower_bound(off);
   if (nx != free_.end() && nx->first == off + len) {

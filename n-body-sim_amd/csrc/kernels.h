@@ -59,7 +59,10 @@ hipError_t sort_pairs(void* tmp, size_t& bytes, const uint64_t* kin,
 // Expands nspans item spans (covering items [0, total) in order) into out.
 hipError_t launch_expand(const ItemSpan* spans, int nspans, int total,
                          WorkItem* out, hipStream_t s);
-hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s);
+// Zeroes bodies [off, off + n) of the arrays named by what (ClearWhat).
+enum ClearWhat : int { CLEAR_RETIRED = 0, CLEAR_ACC = 1 };
+hipError_t launch_clear_range(const Arena* A, int64_t off, int64_t n, int what,
+                              hipStream_t s);
 hipError_t launch_probe(int* out, hipStream_t s);
 
 }  // namespace nbody

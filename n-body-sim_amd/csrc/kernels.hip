@@ -297,9 +297,33 @@ __global__ __launch_bounds__(256) void k_expand(const ItemSpan* spans,
   out[i] = w;
 }
 
-__global__ void k_fill_zero(double* p, int64_t n) {
+// Zeroes bodies [off, off + n) of a group of arena arrays in one launch:
+// CLEAR_RETIRED the positions, both snapshots and g*m (a freed range, so
+// stale values can never be read as sources), CLEAR_ACC the acceleration
+// arrays (a new system).
+__global__ __launch_bounds__(256) void k_clear_range(const Arena* Ap,
+                                                     int64_t off, int64_t n,
+                                                     int what) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0;
+  if (i >= n) return;
+  const CArena* A = (const CArena*)Ap;
+  const int64_t k = off + i;
+  if (what == CLEAR_ACC) {
+    A->ax[k] = 0.0;
+    A->ay[k] = 0.0;
+    A->az[k] = 0.0;
+    return;
+  }
+  A->x[k] = 0.0;
+  A->y[k] = 0.0;
+  A->z[k] = 0.0;
+  for (int b = 0; b < 2; ++b) {
+    A->sx[b][k] = 0.0;
+    A->sy[b][k] = 0.0;
+    A->sz[b][k] = 0.0;
+  }
+  A->gm64[k] = 0.0;
+  A->gm32[k] = 0.f;
 }
 
 // Device-side probe run once at context creation (the reference runs a
@@ -530,10 +554,11 @@ hipError_t launch_expand(const ItemSpan* spans, int nspans, int total,
   return hipGetLastError();
 }
 
-hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s) {
+hipError_t launch_clear_range(const Arena* A, int64_t off, int64_t n, int what,
+                              hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((n + 255) / 256)),
-                     dim3(256), 0, s, p, n);
+  hipLaunchKernelGGL(k_clear_range, dim3((unsigned)((n + 255) / 256)),
+                     dim3(256), 0, s, A, off, n, what);
   return hipGetLastError();
 }
 
