@@ -227,8 +227,13 @@ int nbody_step(nbody_ctx *ctx, nbody_bodies *b, int integrator,
  * steps of their own count: the re-evaluated a(n) is summed in the new
  * order, where nbody_step carries the value summed in the old one; the two
  * agree to rounding.)  dt is the step (the same for
- * every stage of a step).  Asynchronous; NBODY_E_INVALID for an unknown
- * stage or a non-finite dt. */
+ * every stage of a step).  Synchronous.  NBODY_E_INVALID for an unknown
+ * stage or a non-finite dt; NBODY_E_STATE when no
+ * nbody_calculate_accelerations ran for these bodies since the last stage or
+ * step, or when the stage is not the one that comes next (a staged step
+ * starts with EULER, VV_DRIFT or RK4_1 and continues in the order above; a
+ * whole step by nbody_step ends any staged step under way), and dt is then
+ * left unchanged. */
 #define NBODY_STAGE_EULER 0     /* v += a dt; x += v dt */
 #define NBODY_STAGE_VV_DRIFT 1  /* keep a as a_n; x += (v dt) + ((0.5 a) dt^2) */
 #define NBODY_STAGE_VV_KICK 2   /* v += (0.5 (a_n + a)) dt */

@@ -432,6 +432,7 @@ int nbody_integrate_stage(nbody_ctx* ctx, nbody_bodies* b, int stage,
             NBODY_E_INVALID, "unknown integrator stage");
     check_dt(dt);
     ctx->use();
+    b->eng->check_stage(b->id, stage);  // before dt changes
     nbody::SystemParams p = b->eng->params(b->id);
     p.dt = dt;
     b->eng->set_params(b->id, p);

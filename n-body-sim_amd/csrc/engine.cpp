@@ -783,6 +783,8 @@ void Engine::finish_steps(const std::vector<int>& ids, int64_t nsteps) {
     Sys& s = sys_[id];
     if (s.p.integ != INTEG_RK4 && (nsteps & 1)) s.cur ^= 1;
     s.steps += nsteps;
+    s.stage_next = -1;  // whole steps end any staged sequence
+    s.acc_fresh = false;
   }
 }
 
@@ -1045,6 +1047,7 @@ void Engine::accelerations(int id, double* ax, double* ay, double* az) {
     run_round(items, combine);
   }
   s.vv_primed = false;  // the carried acceleration was overwritten
+  s.acc_fresh = true;   // a staged integrator step may use it
   if (ax || ay || az) {
     const size_t bytes = sizeof(double) * s.p.n;
     if (ax)
@@ -1063,10 +1066,24 @@ void Engine::accelerations(int id, double* ax, double* ay, double* az) {
   }
 }
 
-void Engine::integrate_stage(int id, int op) {
+void Engine::check_stage(int id, int op) const {
   check_id(id);
   if (op < OP_EULER || op > OP_RK4_4)
     throw Error(NBODY_E_INVALID, "unknown integrator stage");
+  const Sys& s = sys_[id];
+  // the stages of one step come in their order, each after an evaluation
+  // of the accelerations at the positions the previous stage left
+  if (!s.acc_fresh)
+    throw Error(NBODY_E_STATE,
+                "integrator stage without accelerations evaluated since the "
+                "last stage or step");
+  const bool first = op == OP_EULER || op == OP_VV_DRIFT || op == OP_RK4_1;
+  if (s.stage_next >= 0 ? op != s.stage_next : !first)
+    throw Error(NBODY_E_STATE, "integrator stage out of order");
+}
+
+void Engine::integrate_stage(int id, int op) {
+  check_stage(id, op);
   Sys& s = sys_[id];
   upload_table();
   std::vector<WorkItem> items;
@@ -1083,6 +1100,10 @@ void Engine::integrate_stage(int id, int op) {
   s.prepared = false;    // positions moved: rebuild the snapshot
   s.vv_primed = false;   // a carried acceleration no longer applies
   if (op == OP_EULER || op == OP_VV_KICK || op == OP_RK4_4) ++s.steps;
+  s.stage_next = op == OP_VV_DRIFT ? OP_VV_KICK
+                 : (op >= OP_RK4_1 && op < OP_RK4_4) ? op + 1
+                                                     : -1;
+  s.acc_fresh = false;
 }
 
 void Engine::diagnostics(const std::vector<int>& ids, double* out) {

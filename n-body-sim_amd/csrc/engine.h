@@ -167,6 +167,10 @@ class Engine {
   // One integrator stage (StageOp) of one system on its acceleration arrays
   // (the result of accelerations()), with the system's dt.  Stages that move
   // positions invalidate the snapshot; the stages that end a step count it.
+  // check_stage throws NBODY_E_STATE unless accelerations() ran since the
+  // last stage or step and op is the stage that comes next (a first stage --
+  // Euler, VV drift, RK4 stage 1 -- when no staged step is under way).
+  void check_stage(int id, int op) const;
   void integrate_stage(int id, int op);
   // Per-system diagnostics, DIAG_COUNT doubles per listed system.
   void diagnostics(const std::vector<int>& ids, double* out);
@@ -216,6 +220,11 @@ class Engine {
     double ext = 0.0;           // largest coordinate distance from com
     double mmax = 0.0;          // largest |m| (bounds the f32 self term)
     int64_t sorted_at = -1;     // step count of the last body ordering
+    // staged integration (integrate_stage): the stage that must come next
+    // (-1: any first stage of a step) and whether accelerations were
+    // evaluated since the last stage
+    int stage_next = -1;
+    bool acc_fresh = false;
   };
   // MODE_F32R body ordering.  A system's bodies are put in Hilbert order
   // (perm, and the slot-ordered snapshot copy) before its first F32R

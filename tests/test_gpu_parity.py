@@ -677,6 +677,36 @@ def test_integrate_stage_rejects_bad_arguments(gpu):
     db.close()
 
 
+def test_integrate_stage_order_is_enforced(gpu):
+    """A stage needs accelerations evaluated since the last stage or step,
+    and the stages of a staged step come in their order; a rejected stage
+    moves nothing.  A whole step ends a staged step under way."""
+    lib = nb.load_library()
+    b = nb.plummer(200, 3)
+    grav = nb.GpuNewtonParallelGravity(1.0, 1e-4, nb.Precision.F64)
+    db = nb.DeviceBodies(gpu, b)
+    stage = lambda s: lib.nbody_integrate_stage(gpu.handle, db.handle, s, 1e-3)  # noqa: E731
+    assert stage(nb.STAGE_EULER) == nb.E_STATE           # no accelerations yet
+    grav.calculate_accelerations(gpu, db, download=False)
+    before = db.download()
+    assert stage(nb.STAGE_RK4[1]) == nb.E_STATE          # RK4 stage 2 first
+    assert stage(nb.STAGE_VV_KICK) == nb.E_STATE         # kick without drift
+    after = db.download()
+    for f in ("pos_x", "vel_x"):
+        assert np.array_equal(getattr(before, f), getattr(after, f))
+    assert stage(nb.STAGE_RK4[0]) == nb.OK
+    assert stage(nb.STAGE_RK4[1]) == nb.E_STATE          # needs a new evaluation
+    grav.calculate_accelerations(gpu, db, download=False)
+    assert stage(nb.STAGE_RK4[2]) == nb.E_STATE          # skips stage 2
+    assert stage(nb.STAGE_RK4[1]) == nb.OK
+    # a whole step in between ends the staged RK4 step
+    nb.IntegratorMethod.EULER.gpu_create(1e-3).step(gpu, db, grav, 1)
+    grav.calculate_accelerations(gpu, db, download=False)
+    assert stage(nb.STAGE_RK4[2]) == nb.E_STATE
+    assert stage(nb.STAGE_RK4[0]) == nb.OK
+    db.close()
+
+
 def _with_variant_tables(tables, fn):
     """Runs fn with the given variant tables ({precision: [32 entries]}),
     then restores the loaded ones."""
