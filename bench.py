@@ -190,7 +190,8 @@ def roofline(dtype, interactions, kt, step_rate=None, traffic=None):
 F32_FORMS = {"f32r": nb.Precision.F32R, "f32c": nb.Precision.F32C}
 PRECS = dict(F32_FORMS, f64=nb.Precision.F64)
 DTYPE = {
-    "f32r": "f32 pair terms (per-wave relative coordinates; f64 differences "
+    "f32r": "f32 pair terms (per-wave relative coordinates; r^2 of far chunks "
+            "from bf16-split Gram products on the matrix cores; f64 differences "
             "split in two f32 words for near chunks), f64 state",
     "f32c": "f32 pair terms (f64 differences), f64 state",
     "f64": "f64",

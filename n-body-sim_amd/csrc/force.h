@@ -374,18 +374,43 @@ __device__ __forceinline__ float uniform_f32(float v) {  // into an SGPR
       float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int32_t, v)));
 }
 
-// True unless the bound R + |s| <= K * D holds for source s (see above).
-// Overflow or NaN in the test makes the source near (the exact path).  The
-// test runs in f32 with v_sqrt_f32 (about 1 ulp); its rounding moves K by a
-// few parts in 10^7.
-__device__ __forceinline__ bool near_source(const TargetR& t, float sx,
-                                            float sy, float sz) {
+// Source classes by the bound R + |s| <= K * D (see above): near unless it
+// holds at K = kNearK; matrix-core class (far enough for r^2 from the Gram
+// form on the matrix cores, below) when it holds at K = kMatK, D^2 >= 2^-60,
+// and the source's terms stay in range there: g max(|s|, R, 1) <= 2^100 D^3
+// (so w g, w g |s| and R w g stay below 2^100 with w = r^-3 <= D^-3) and
+// g |s| <= 2^120 (the staged g s).  Overflow or NaN in a test makes the
+// source near (the exact path) or keeps it out of the matrix-core class.
+// The tests run in f32 with v_sqrt_f32 (about 1 ulp); its rounding moves K
+// by a few parts in 10^7.
+struct SrcClass {
+  bool near, mat;
+};
+
+#ifndef NBODY_F32R_MATRIX
+#define NBODY_F32R_MATRIX 1
+#endif
+constexpr float kMatK = 3.0f;
+constexpr float kMatMinD2 = 0x1p-60f;
+
+__device__ __forceinline__ SrcClass classify_source(const TargetR& t, float sx,
+                                                    float sy, float sz,
+                                                    float g) {
   const float ex = fmaxf(fmaxf(t.lox - sx, sx - t.hix), 0.f);
   const float ey = fmaxf(fmaxf(t.loy - sy, sy - t.hiy), 0.f);
   const float ez = fmaxf(fmaxf(t.loz - sz, sz - t.hiz), 0.f);
   const float d2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-  const float a = t.rad + __builtin_amdgcn_sqrtf(fmaf(sz, sz, fmaf(sy, sy, sx * sx)));
-  return !(a * a <= (kNearK * kNearK) * d2);
+  const float sn = __builtin_amdgcn_sqrtf(fmaf(sz, sz, fmaf(sy, sy, sx * sx)));
+  const float a = t.rad + sn;
+  const float a2 = a * a;
+  const bool near = !(a2 <= (kNearK * kNearK) * d2);
+  if (!NBODY_F32R_MATRIX) return {near, false};
+  const float d3 = d2 * __builtin_amdgcn_sqrtf(d2);
+  const float gs = g * sn;
+  const bool mat = (a2 <= (kMatK * kMatK) * d2) && (d2 >= kMatMinD2) &&
+                   (g * fmaxf(fmaxf(sn, t.rad), 1.f) <= 0x1p100f * d3) &&
+                   (gs <= 0x1p120f);
+  return {near, mat};
 }
 
 // The value in a VGPR.  Uniform values otherwise stay in SGPRs, and on
@@ -396,6 +421,163 @@ __device__ __forceinline__ float in_vgpr(float v) {
   return __builtin_bit_cast(
       float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xe4, 0xf,
                                       0xf, false));
+}
+
+// ---- MODE_F32R, matrix-core class ---------------------------------------
+//
+// For a chunk whose every source is far at K = kMatK, r^2 comes from the Gram
+// form r^2 = q + p - 2 s.t (q = |s|^2 + eps^2 per source, p = |t|^2 per
+// target) on v_mfma_f32_16x16x32_bf16.  Each f32 value is split exactly into
+// three bf16 pieces, v = v1 + v2 + v3 (round to nearest each: |v2| <= 2^-8 |v|,
+// |v3| <= 2^-16 |v|), and the K = 32 slots of one source and one target hold
+// the products v_a * w_b with a + b <= 4 of each coordinate (the dropped ones
+// are below 2^-24 |s||t|), q's pieces against 1 and 1 against p's pieces.
+// The matrix core forms each product exactly and adds in f32, rounding at
+// every addition (measured), so r^2 carries at most about 31 roundings of
+// quantities below (|s| + |t|)^2 + eps^2 <= K^2 r^2: a relative error below
+// 31 K^2 2^-24 = 1.7e-5 in the worst case, with typical errors near 2^-24.
+//
+// Per 16-source block and per group c of 16 targets, one instruction gives
+// lane (grp = l/16, col = l%16) r^2 of targets 16c + col against sources
+// 16b + 4grp + r in its four result registers; the lane then adds
+// w (g s, g) with w = r^-3 into its four-wide sums of target 16c + col: 7 VALU
+// instructions per pair (v_rsq_f32, two multiplies, four FMAs) against 13 in
+// the f32 loop, and the matrix instructions overlap the VALU.  At the end of
+// the chunk the four lane groups' sums of each target are added through LDS,
+// and a_i gains (sum w g s) - t_i (sum w g).
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr uint32_t kBf16One = 0x3f80u;  // 1.0 in bf16
+
+// bf16 of a (low half) and b (high half), each rounded to nearest even
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+  typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+  const bf16x2_t v = {(__bf16)a, (__bf16)b};
+  return __builtin_bit_cast(uint32_t, v);
+}
+__device__ __forceinline__ float bf16_low(uint32_t p) {
+  return __builtin_bit_cast(float, p << 16);
+}
+__device__ __forceinline__ float bf16_high(uint32_t p) {
+  return __builtin_bit_cast(float, p & 0xffff0000u);
+}
+
+// a = a1 + a2 + a3 and b = b1 + b2 + b3 exactly, packed pairwise: h = (a1, b1),
+// m = (a2, b2), l = (a3, b3).  Each residual is exact in f32 (Sterbenz), and
+// the last has at most 8 significant bits, so it is exact in bf16 (values
+// within f32's normal range).
+__device__ __forceinline__ void split3(float a, float b, uint32_t& h,
+                                       uint32_t& m, uint32_t& l) {
+  h = pack_bf16(a, b);
+  const float ra = a - bf16_low(h), rb = b - bf16_high(h);
+  m = pack_bf16(ra, rb);
+  l = pack_bf16(ra - bf16_low(m), rb - bf16_high(m));
+}
+
+// The lane's target as the B operands of the four target groups: lane (grp,
+// col) holds K dwords 4grp..4grp+3 of target 16c + col.  Target dwords
+// (pieces of -2t, pieces of p = |t|^2):
+//   (x1,y1) (x2,y2) (x3,y3) (x1,y1) | (x2,y2) (x1,y1) (z1,1) (z2,0) |
+//   (z3,0) (z1,1) (z2,0) (z1,1) | (p1,p2) (p3,0) 0 0
+// Source dwords (pieces of s, of q = |s|^2 + eps^2), slot for slot:
+//   (x1,y1) (x1,y1) (x1,y1) (x2,y2) | (x2,y2) (x3,y3) (z1,q1) (z1,q1) |
+//   (z1,q1) (z2,q2) (z2,q2) (z3,q3) | (1,1) (1,0) 0 0
+// The last source quad is the same for every source and stays in LDS once.
+// kq: the wave's quad area (at least 257 quads); leaves quad 256 holding it.
+__device__ __forceinline__ void mat_targets(u32x4_t* kq, const TargetR& t,
+                                            u32x4_t (&Bt)[4]) {
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  const int grp = lane >> 4, col = lane & 15;
+  uint32_t xh, xm, xl, zh, zm, zl;
+  split3(-2.f * t.x, -2.f * t.y, xh, xm, xl);
+  split3(-2.f * t.z, fmaf(t.z, t.z, fmaf(t.y, t.y, t.x * t.x)), zh, zm, zl);
+  const uint32_t z1 = (zh & 0xffffu) | (kBf16One << 16), z2 = zm & 0xffffu,
+                 z3 = zl & 0xffffu;
+  kq[4 * lane + 0] = u32x4_t{xh, xm, xl, xh};
+  kq[4 * lane + 1] = u32x4_t{xm, xh, z1, z2};
+  kq[4 * lane + 2] = u32x4_t{z3, z1, z2, z1};
+  kq[4 * lane + 3] = u32x4_t{(zh >> 16) | (zm & 0xffff0000u), zl >> 16, 0u, 0u};
+  wave_sync();
+#pragma unroll
+  for (int c = 0; c < 4; ++c) Bt[c] = kq[4 * (16 * c + col) + grp];
+  wave_sync();
+  if (lane == 0) kq[256] = u32x4_t{kBf16One | (kBf16One << 16), kBf16One, 0u, 0u};
+}
+
+// Stages the lane's source (s, q, g) for the matrix-core class: its three
+// K quads at kq[3 lane ..] and (g s, g) at rec[lane].
+__device__ __forceinline__ void mat_stage(u32x4_t* kq, float4* rec, float sx,
+                                          float sy, float sz, float g,
+                                          float eps2) {
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  const float q = fmaf(sz, sz, fmaf(sy, sy, fmaf(sx, sx, eps2)));
+  uint32_t h, m, l, zh, zm, zl;
+  split3(sx, sy, h, m, l);
+  split3(sz, q, zh, zm, zl);
+  kq[3 * lane + 0] = u32x4_t{h, h, h, m};
+  kq[3 * lane + 1] = u32x4_t{m, l, zh, zh};
+  kq[3 * lane + 2] = u32x4_t{zh, zm, zm, zl};
+  rec[lane] = {g * sx, g * sy, g * sz, g};
+}
+
+// One staged chunk of 64 sources (matrix-core class) against the wave's 64
+// targets; adds the lane's own target's sum to acc.  Reads kq quads
+// abase + 48 b (lane groups 0-2: their source's quad; group 3: the constant
+// quad 256) and rec; then reuses kq[0, 256) for the cross-lane sums.
+__device__ __forceinline__ void chunk_f32r_mat(u32x4_t* kq, const float4* rec,
+                                               const u32x4_t (&Bt)[4],
+                                               const TargetR& t, Kahan3& acc) {
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  const int grp = lane >> 4, col = lane & 15;
+  const int abase = grp < 3 ? 3 * col + grp : 256;
+  const int astride = grp < 3 ? 48 : 0;
+  f32x4_t a[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) a[c] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int b = 0; b < 4; ++b) {
+    const bf16x8_t As = __builtin_bit_cast(bf16x8_t, kq[abase + astride * b]);
+    f32x4_t r2[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      r2[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+          As, __builtin_bit_cast(bf16x8_t, Bt[c]), f32x4_t{0.f, 0.f, 0.f, 0.f},
+          0, 0, 0);
+    // one source record at a time, against the four target groups
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float4 sd = rec[16 * b + 4 * grp + r];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float inv = __builtin_amdgcn_rsqf(r2[c][r]);
+        const float w = (inv * inv) * inv;
+        a[c][0] = fmaf(w, sd.x, a[c][0]);
+        a[c][1] = fmaf(w, sd.y, a[c][1]);
+        a[c][2] = fmaf(w, sd.z, a[c][2]);
+        a[c][3] = fmaf(w, sd.w, a[c][3]);
+      }
+    }
+  }
+  wave_sync();  // every quad read before the sums overwrite them
+  float4* red = (float4*)kq;
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    red[4 * (16 * c + col) + grp] = __builtin_bit_cast(float4, a[c]);
+  wave_sync();
+  float4 f = red[4 * lane];
+#pragma unroll
+  for (int g2 = 1; g2 < 4; ++g2) {
+    const float4 v = red[4 * lane + g2];
+    f.x += v.x;
+    f.y += v.y;
+    f.z += v.z;
+    f.w += v.w;
+  }
+  kahan_add(acc.sx, acc.cx, fmaf(-t.x, f.w, f.x));
+  kahan_add(acc.sy, acc.cy, fmaf(-t.y, f.w, f.y));
+  kahan_add(acc.sz, acc.cz, fmaf(-t.z, f.w, f.z));
 }
 
 // One staged chunk of 64 sources against the lane's target.  RES adds the
@@ -459,6 +641,7 @@ __device__ __forceinline__ void chunk_f32r(const float4* stage,
 // as zero mass at the origin.
 template <bool NOMASK, int BATCH>
 __device__ __forceinline__ void sweep_f32r(float4* stage, float4* res,
+                                           u32x4_t* kq, const u32x4_t (&Bt)[4],
                                            const HSrc* H, int64_t jlo,
                                            int64_t jhi,
                                            int64_t end, double ox, double oy,
@@ -478,9 +661,18 @@ __device__ __forceinline__ void sweep_f32r(float4* stage, float4* res,
       g = h.g;
     }
     const float sx = (float)dx, sy = (float)dy, sz = (float)dz;
+    const SrcClass cl = classify_source(t, sx, sy, sz, g);
+    // wave-uniform: every source valid and in the matrix-core class
+    const bool all_mat = __ballot(!(valid && cl.mat)) == 0;
+    if (all_mat) {
+      mat_stage(kq, stage, sx, sy, sz, g, eps2);
+      wave_sync();
+      chunk_f32r_mat(kq, stage, Bt, t, acc);
+      wave_sync();
+      continue;
+    }
     stage[lane] = {sx, sy, sz, g};
-    const bool near = valid && near_source(t, sx, sy, sz);
-    const bool any_near = __ballot(near) != 0;  // wave-uniform
+    const bool any_near = __ballot(valid && cl.near) != 0;  // wave-uniform
     if (any_near)
       res[lane] = {(float)(dx - (double)sx), (float)(dy - (double)sy),
                    (float)(dz - (double)sz), 0.f};
@@ -519,7 +711,12 @@ __global__ __launch_bounds__(kWave * W) void k_force(const Arena* Ap,
 
   const int nsplit = sd->nsplit, part = it.part;
   if (MODE == MODE_F32R) {
-    __shared__ float4 stage[W][2][kWave];  // staged s, residuals
+    // per wave: 64 staged sources, then an area holding either the near
+    // chunks' residuals (64) or the matrix-core class's K quads and cross-lane
+    // sums (257 quads)
+    __shared__ float4 lds[W][kWave + 257];
+    float4* stage = lds[wave];
+    u32x4_t* kq = (u32x4_t*)(lds[wave] + kWave);
     const int64_t ts = active ? slot : end - 1;  // in this tile
     const HSrc h0 = A->hs[src][ts];
     const double px0 = h0.x, py0 = h0.y, pz0 = h0.z;
@@ -542,17 +739,19 @@ __global__ __launch_bounds__(kWave * W) void k_force(const Arena* Ap,
     t.hiz = uniform_f32(wave_max(t.z));
     t.rad = uniform_f32(wave_max(
         __builtin_amdgcn_sqrtf(fmaf(t.z, t.z, fmaf(t.y, t.y, t.x * t.x)))));
+    u32x4_t Bt[4];
+    mat_targets(kq, t, Bt);
     Kahan3 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     int64_t jlo, jhi;
     part_range<kWave>(off, n, nsplit, part, jlo, jhi);
     const float e2 = sd->eps2f;
     constexpr int B = Shape<S>::B;
     if (sd->flags & SYS_F32_SELF_FINITE)
-      sweep_f32r<true, B>(stage[wave][0], stage[wave][1], A->hs[src], jlo,
-                          jhi, end, ox, oy, oz, t, e2, acc);
+      sweep_f32r<true, B>(stage, (float4*)kq, kq, Bt, A->hs[src], jlo, jhi,
+                          end, ox, oy, oz, t, e2, acc);
     else
-      sweep_f32r<false, B>(stage[wave][0], stage[wave][1], A->hs[src], jlo,
-                           jhi, end, ox, oy, oz, t, e2, acc);
+      sweep_f32r<false, B>(stage, (float4*)kq, kq, Bt, A->hs[src], jlo, jhi,
+                           end, ox, oy, oz, t, e2, acc);
     ax = (double)acc.sx - (double)acc.cx;
     ay = (double)acc.sy - (double)acc.cy;
     az = (double)acc.sz - (double)acc.cz;

@@ -1,0 +1,325 @@
+"""CPU model of MODE_F32R's matrix-core class (n-body-sim_amd/csrc/force.h:
+classify_source, split3, mat_targets, mat_stage, chunk_f32r_mat).
+
+For a 64-source chunk whose every source is far at K = 3 (R + |s| <= 3 D),
+r^2 = q + p - 2 s.t comes from v_mfma_f32_16x16x32_bf16: each f32 value is
+split into three bf16 pieces, the 32 K slots of a source and a target hold
+the piece products of each coordinate with a + b <= 4, q's pieces against 1
+and 1 against p's pieces, and the matrix core forms each product exactly and
+adds them in f32 (rounding at every addition; the order is the hardware's).
+
+This replays that arithmetic in numpy float32 with the kernel's slot layout
+and checks what the kernel's accuracy rests on:
+  * the three-piece split is exact and the slots form q + p - 2 s.t up to the
+    dropped piece products (a transposed or shifted slot fails at O(1));
+  * at the class boundary, r^2 summed in f32 in any order (slot order,
+    reversed, by magnitude either way, a pairwise tree, random) stays within
+    31 K^2 2^-24 of exact;
+  * accelerations of the three-class sweep stay within 1e-5 of the oracle on
+    a Plummer sphere and on tight pairs in a halo, with the class carrying a
+    fifth of the N = 4096 Plummer sphere's chunks;
+  * the range guard keeps out exactly the sources whose staged g s or whose
+    terms would leave f32 (astronomical SI units).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from test_f32r_model import halo_with_tight_pairs, hilbert_order
+
+F = np.float32
+U = 2.0 ** -24
+KM = 3.0        # kMatK
+KN = 8.0        # kNearK
+BOUND = 31 * KM * KM * U
+
+
+def bf16(x):
+    """Round float32 values to bf16 (nearest, ties to even), as float32."""
+    u = np.asarray(x, F).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(F)
+
+
+def split3(x):
+    x = np.asarray(x, F)
+    h = bf16(x)
+    r1 = (x - h).astype(F)
+    m = bf16(r1)
+    lo = bf16((r1 - m).astype(F))
+    return h, m, lo
+
+
+def source_slots(s, q):
+    """The 32 K slots of each source (mat_stage): dwords (low, high)
+    (x1,y1) x3, (x2,y2) x2, (x3,y3), (z1,q1) x3, (z2,q2) x2, (z3,q3),
+    (1,1), (1,0), 0, 0."""
+    x1, x2, x3 = split3(s[:, 0])
+    y1, y2, y3 = split3(s[:, 1])
+    z1, z2, z3 = split3(s[:, 2])
+    q1, q2, q3 = split3(q)
+    one, zero = np.ones_like(q1), np.zeros_like(q1)
+    dw = [(x1, y1), (x1, y1), (x1, y1), (x2, y2), (x2, y2), (x3, y3),
+          (z1, q1), (z1, q1), (z1, q1), (z2, q2), (z2, q2), (z3, q3),
+          (one, one), (one, zero), (zero, zero), (zero, zero)]
+    return np.stack([v for d in dw for v in d], 1).astype(F)
+
+
+def target_slots(t):
+    """The 32 K slots of each target (mat_targets), pieces of u = -2t and of
+    p = |t|^2: (ux1,uy1) (ux2,uy2) (ux3,uy3) (ux1,uy1) (ux2,uy2) (ux1,uy1)
+    (uz1,1) (uz2,0) (uz3,0) (uz1,1) (uz2,0) (uz1,1) (p1,p2) (p3,0) 0 0."""
+    t = np.asarray(t, F)
+    u = (F(-2.0) * t).astype(F)
+    p = (t[:, 2] * t[:, 2] + (t[:, 1] * t[:, 1] + t[:, 0] * t[:, 0]).astype(F)).astype(F)
+    x1, x2, x3 = split3(u[:, 0])
+    y1, y2, y3 = split3(u[:, 1])
+    z1, z2, z3 = split3(u[:, 2])
+    p1, p2, p3 = split3(p)
+    one, zero = np.ones_like(p1), np.zeros_like(p1)
+    dw = [(x1, y1), (x2, y2), (x3, y3), (x1, y1), (x2, y2), (x1, y1),
+          (z1, one), (z2, zero), (z3, zero), (z1, one), (z2, zero), (z1, one),
+          (p1, p2), (p3, zero), (zero, zero), (zero, zero)]
+    return np.stack([v for d in dw for v in d], 1).astype(F)
+
+
+def q_of(s, eps2):
+    s = np.asarray(s, F)
+    return (s[:, 2] * s[:, 2] + (s[:, 1] * s[:, 1] +
+            (s[:, 0] * s[:, 0] + F(eps2)).astype(F)).astype(F)).astype(F)
+
+
+def gram_r2(SS, TT, order="slot"):
+    """r^2 of every (target, source) pair: the slot products (exact in f32)
+    added in float32 in the given order."""
+    P = (TT[:, None, :] * SS[None, :, :]).astype(F)
+    k = P.shape[2]
+    if order == "tree":
+        while P.shape[2] > 1:
+            if P.shape[2] % 2:
+                P = np.concatenate([P, np.zeros_like(P[:, :, :1])], 2)
+            P = (P[:, :, 0::2] + P[:, :, 1::2]).astype(F)
+        return P[:, :, 0]
+    if order == "slot":
+        idx = np.broadcast_to(np.arange(k), P.shape)
+    elif order == "reverse":
+        idx = np.broadcast_to(np.arange(k)[::-1], P.shape)
+    elif order == "ascending":
+        idx = np.argsort(np.abs(P), axis=2)
+    elif order == "descending":
+        idx = np.argsort(-np.abs(P), axis=2)
+    else:   # a seeded random order per pair
+        rng = np.random.default_rng(int(order))
+        idx = np.argsort(rng.random(P.shape), axis=2)
+    P = np.take_along_axis(P, idx, 2)
+    acc = P[:, :, 0].copy()
+    for j in range(1, k):
+        acc = (acc + P[:, :, j]).astype(F)
+    return acc
+
+
+ORDERS = ["slot", "reverse", "ascending", "descending", "tree", "1", "2"]
+
+
+def test_split3_exact():
+    rng = np.random.default_rng(1)
+    x = (rng.uniform(1, 2, 20000) * 2.0 ** rng.integers(-100, 100, 20000) *
+         rng.choice([-1, 1], 20000)).astype(F)
+    x = np.concatenate([x, F([1.0, -1.0, 2.0 ** -100, 3.0, 1 - 2.0 ** -24, 1 + 2.0 ** -23])])
+    h, m, lo = split3(x)
+    assert np.array_equal(h.astype(np.float64) + m + lo, x.astype(np.float64))
+    for piece in (h, m, lo):   # every piece is a bf16 value
+        assert not (piece.view(np.uint32) & 0xFFFF).any()
+    ax = np.abs(x.astype(np.float64))
+    assert (np.abs(m) <= 2.0 ** -8 * ax).all()
+    assert (np.abs(lo) <= 2.0 ** -16 * ax).all()
+
+
+def test_slots_form_the_gram_r2():
+    """Exact sums of the slot products equal q + p - 2 s.t up to the dropped
+    piece products (below 2^-24 |s||t| per coordinate, three coordinates)."""
+    rng = np.random.default_rng(2)
+    s = rng.standard_normal((64, 3)).astype(F) * F(3)
+    t = rng.standard_normal((64, 3)).astype(F) * F(0.2)
+    eps2 = 1e-4
+    q = q_of(s, eps2)
+    SS, TT = source_slots(s, q).astype(np.float64), target_slots(t).astype(np.float64)
+    got = TT @ SS.T
+    p = (t.astype(np.float64) ** 2).sum(1)
+    want = q.astype(np.float64)[None, :] + p[:, None] - 2 * t.astype(np.float64) @ s.T.astype(np.float64)
+    scale = np.linalg.norm(t, axis=1)[:, None] * np.linalg.norm(s, axis=1)[None, :]
+    # p is rounded once to f32 in the kernel (target_slots splits the f32 p)
+    perr = 2 * U * p[:, None]
+    assert (np.abs(got - want) <= 6 * U * scale + perr + 1e-300).all()
+    # a slot layout that pairs the wrong pieces is off at O(1): x against y,
+    # or z against the slot that carries q
+    for a, b in ((0, 1), (12, 13)):
+        bad = TT.copy()
+        bad[:, [a, b]] = bad[:, [b, a]]
+        assert np.abs(bad @ SS.T - want).max() > 1e-3 * want.max(), (a, b)
+
+
+def boundary_geometry(seed, n_t=64, n_s=64):
+    """Targets in a small box around the origin and sources placed so that
+    (R + |s|)^2 sits at 9 D^2 (the class boundary), in random directions."""
+    rng = np.random.default_rng(seed)
+    t = (rng.uniform(-1, 1, (n_t, 3)) * [1.0, 0.6, 0.3]).astype(F)
+    t[n_t // 2] = 0      # the wave's origin is its middle target
+    lo, hi = t.min(0), t.max(0)
+    rad = float(np.sqrt((t.astype(np.float64) ** 2).sum(1)).max())
+    s = []
+    while len(s) < n_s:
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        # grow |s| until R + |s| <= 3 D holds, then keep the first such point
+        for rr in np.geomspace(rad, 1e3 * rad, 2000):
+            p = d * rr
+            e = np.maximum(np.maximum(lo - p, p - hi), 0)
+            D = np.linalg.norm(e)
+            if D > 0 and rad + rr <= KM * D:
+                s.append(p)
+                break
+    return t, np.array(s, F)
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_gram_r2_within_bound_at_the_class_boundary(seed):
+    t, s = boundary_geometry(seed)
+    for eps2 in (0.0, 1e-6):
+        q = q_of(s, eps2)
+        SS, TT = source_slots(s, q), target_slots(t)
+        d = s.astype(np.float64)[None, :, :] - t.astype(np.float64)[:, None, :]
+        exact = (d * d).sum(2) + float(F(eps2))
+        worst = {}
+        for order in ORDERS:
+            rel = np.abs(gram_r2(SS, TT, order).astype(np.float64) - exact) / exact
+            worst[order] = rel.max()
+            assert rel.max() <= BOUND, (order, rel.max(), BOUND)
+        # the rounding of p, q and s itself and ~31 additions: typical errors
+        # stay a few units of 2^-24 of the pair's own r^2
+        rel = np.abs(gram_r2(SS, TT, "slot").astype(np.float64) - exact) / exact
+        assert np.median(rel) <= 8 * U, np.median(rel)
+        print(f"seed {seed} eps2 {eps2}: worst relative r^2 error by order "
+              + ", ".join(f"{k} {v / U:.1f}u" for k, v in worst.items())
+              + f" (bound {BOUND / U:.0f}u)")
+
+
+def classify(S, si, g):
+    """classify_source in float32: (near, mat) per source for one wave."""
+    with np.errstate(over="ignore", invalid="ignore"):
+        lo, hi = si.min(0), si.max(0)
+        rad = np.sqrt((si * si).sum(1, dtype=F)).max()
+        ev = np.maximum(np.maximum(lo - S, S - hi), F(0))
+        d2 = (ev * ev).sum(1, dtype=F)
+        sn = np.sqrt((S * S).sum(1, dtype=F)).astype(F)
+        a = (rad + sn).astype(F)
+        a2 = (a * a).astype(F)
+        near = ~(a2 <= F(KN * KN) * d2)
+        d3 = (d2 * np.sqrt(d2)).astype(F)
+        gs = (g * sn).astype(F)
+        mat = ((a2 <= F(KM * KM) * d2) & (d2 >= F(2.0 ** -60)) &
+               ((g * np.maximum(np.maximum(sn, rad), F(1))).astype(F) <= F(2.0 ** 100) * d3) &
+               (gs <= F(2.0 ** 120)))
+    return near, mat
+
+
+def model_sweep(m, P, eps2, g=1.0):
+    """Accelerations of every body (slot order) with MODE_F32R's three chunk
+    classes; per-pair terms summed in f64, so the model checks the pair
+    arithmetic of each class.  Returns (acc, share of matrix-core chunks)."""
+    n = len(m)
+    gm = (g * m).astype(F)
+    acc = np.zeros((n, 3))
+    nmat = nchunk = 0
+    for t0 in range(0, n, 64):
+        T = P[t0:t0 + 64]
+        o = T[min(32, len(T) - 1)]
+        S = (P - o).astype(F)
+        R = ((P - o) - S.astype(np.float64)).astype(F)
+        si, ri = S[t0:t0 + 64], R[t0:t0 + 64]
+        near_s, mat_s = classify(S, si, gm)
+        for c0 in range(0, n, 64):
+            sl = slice(c0, min(c0 + 64, n))
+            full = sl.stop - c0 == 64
+            nchunk += 1
+            if full and mat_s[sl].all():
+                nmat += 1
+                q = q_of(S[sl], eps2)
+                r2 = gram_r2(source_slots(S[sl], q), target_slots(si), "slot")
+                inv = (F(1) / np.sqrt(r2)).astype(F)
+                w = ((inv * inv) * inv).astype(F)
+                rec = (gm[sl, None] * S[sl]).astype(F)
+                sw = w.astype(np.float64) @ gm[sl].astype(np.float64)
+                acc[t0:t0 + 64] += (w.astype(np.float64) @ rec.astype(np.float64)
+                                    - si.astype(np.float64) * sw[:, None])
+                continue
+            d = (S[None, sl, :] - si[:, None, :]).astype(F)
+            if near_s[sl].any():
+                d = (d + (R[None, sl, :] - ri[:, None, :]).astype(F)).astype(F)
+            r2 = ((d * d).sum(2, dtype=F) + F(eps2)).astype(F)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                inv = (F(1) / np.sqrt(r2)).astype(F)
+                k = (gm[None, sl] * ((inv * inv) * inv)).astype(F)
+                k[np.abs(d).sum(2) == 0] = 0        # a target's own pair
+            acc[t0:t0 + 64] += np.einsum("ij,ijk->ik", k.astype(np.float64),
+                                         d.astype(np.float64))
+    return acc, nmat / nchunk
+
+
+def plummer(n, seed):
+    rng = np.random.default_rng(seed)
+    r = 1.0 / np.sqrt(rng.uniform(1e-3, 1, n) ** (-2.0 / 3.0) - 1.0)
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    p = d * r[:, None]
+    return np.full(n, 1.0 / n), p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy()
+
+
+@pytest.mark.parametrize("geometry", ["plummer", "halo"])
+def test_three_class_sweep_within_1e5(geometry):
+    m, x, y, z = plummer(4096, 5) if geometry == "plummer" else halo_with_tight_pairs()
+    perm = hilbert_order(x, y, z, m)
+    P = np.stack([x, y, z], 1)[perm]
+    mp = m[perm]
+    eps = 0.005 if geometry == "plummer" else 1e-6
+    ref = np.stack(oracle.newton_parallel(mp, P[:, 0].copy(), P[:, 1].copy(),
+                                          P[:, 2].copy(), 1.0, eps), 1)
+    acc, share = model_sweep(mp, P, eps * eps)
+    e = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    print(f"{geometry}: matrix-core chunks {share:.3f}, max rel {e.max():.2e}")
+    assert e.max() <= 1e-5, e.max()
+    if geometry == "plummer":   # 0.21 at N = 4096 (0.88 at N = 65536)
+        assert share > 0.15, share
+
+
+def test_range_guard_keeps_out_overflowing_sources():
+    """Astronomical SI units: positions ~3e17 m, g*m = 1e30 per body.  The
+    staged g s (~3e47) would overflow f32; the guard keeps those sources out
+    of the matrix-core class, and every source it admits has every term of
+    the class (g s, w g, w g |s|, R w g) below 2^120."""
+    rng = np.random.default_rng(9)
+    si = (rng.uniform(-1, 1, (64, 3)) * 1e15).astype(F)
+    si[32] = 0
+    S = (rng.standard_normal((4096, 3)) * 3e17).astype(F)
+    for gval, expect_any in ((F(1e30), False), (F(1e-3), True)):
+        g = np.full(len(S), gval, F)
+        _, mat = classify(S, si, g)
+        assert mat.any() == expect_any, (gval, mat.sum())
+        if mat.any():
+            with np.errstate(over="ignore"):
+                rad = np.sqrt((si.astype(np.float64) ** 2).sum(1)).max()
+                sn = np.linalg.norm(S[mat].astype(np.float64), axis=1)
+                lo, hi = si.min(0).astype(np.float64), si.max(0).astype(np.float64)
+                e = np.maximum(np.maximum(lo - S[mat], S[mat] - hi), 0)
+                D = np.linalg.norm(e, axis=1)
+                w = D ** -3.0
+                gg = g[mat].astype(np.float64)
+                for term in (gg * sn, w * gg, w * gg * sn, rad * w * gg):
+                    assert (term < 2.0 ** 120).all()
+    # without the guard, the 1e30 case passes the geometric test and
+    # overflows the staged g s
+    _, mat_geom = classify(S, si, np.zeros(len(S), F))
+    assert mat_geom.any()
+    with np.errstate(over="ignore"):
+        gs = (F(1e30) * np.sqrt((S * S).sum(1, dtype=F))).astype(F)
+    assert np.isinf(gs[mat_geom]).all()

@@ -581,6 +581,27 @@ def test_system_far_from_the_origin(gpu, shift):
         assert e.max() <= tol, prec.name
 
 
+def test_f32r_astronomical_units(gpu):
+    """Large units: a 4096-body Plummer sphere scaled to 1e12 length units
+    with g*m = 1e27 per body.  F32R's matrix-core class stages g*s per
+    source, which would reach ~1e39 and overflow f32; its range guard
+    (g |s| <= 2^120) sends those chunks to the f32 loop, which forms
+    g*m*r^-3 (~1e-9) first and stays in range.  F32R stays finite and within
+    1e-5 of the oracle, as F32C does."""
+    b = nb.plummer(4096, 37)
+    L = 1e12
+    b = nb.Bodies(np.full(4096, 1e27), b.pos_x * L, b.pos_y * L, b.pos_z * L,
+                  b.vel_x, b.vel_y, b.vel_z)
+    eps = 0.005 * L
+    ref = oracle.newton_parallel(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, eps)
+    for prec in (nb.Precision.F32R, nb.Precision.F32C):
+        got = gpu_acc(gpu, b, 1.0, eps, prec)
+        assert all(np.isfinite(a).all() for a in got), prec.name
+        e = rel_err(got, ref)
+        print(f"{prec.name}: max {e.max():.3e}", flush=True)
+        assert e.max() <= 1e-5, (prec.name, e.max())
+
+
 def test_kernel_timing_counts_and_bits(gpu):
     """Launch timing brackets every force and combine launch with events:
     five velocity-Verlet steps are five force launches (one combine each when
