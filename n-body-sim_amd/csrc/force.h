@@ -390,7 +390,7 @@ struct SrcClass {
 #ifndef NBODY_F32R_MATRIX
 #define NBODY_F32R_MATRIX 1
 #endif
-constexpr float kMatK = 3.0f;
+constexpr float kMatK = 4.0f;
 constexpr float kMatMinD2 = 0x1p-60f;
 
 __device__ __forceinline__ SrcClass classify_source(const TargetR& t, float sx,
@@ -435,16 +435,18 @@ __device__ __forceinline__ float in_vgpr(float v) {
 // The matrix core forms each product exactly and adds in f32, rounding at
 // every addition (measured), so r^2 carries at most about 31 roundings of
 // quantities below (|s| + |t|)^2 + eps^2 <= K^2 r^2: a relative error below
-// 31 K^2 2^-24 = 1.7e-5 in the worst case, with typical errors near 2^-24.
+// 31 K^2 2^-24 = 3.0e-5 at K = 4 in the worst case.  Sources placed at the
+// class boundary give at most 26 units of 2^-24 in any summation order
+// (tests/test_f32r_matrix_model.py), and typical pairs a few units.
 //
 // Per 16-source block and per group c of 16 targets, one instruction gives
 // lane (grp = l/16, col = l%16) r^2 of targets 16c + col against sources
 // 16b + 4grp + r in its four result registers; the lane then adds
 // w (g s, g) with w = r^-3 into its four-wide sums of target 16c + col: 7 VALU
 // instructions per pair (v_rsq_f32, two multiplies, four FMAs) against 13 in
-// the f32 loop, and the matrix instructions overlap the VALU.  At the end of
-// the chunk the four lane groups' sums of each target are added through LDS,
-// and a_i gains (sum w g s) - t_i (sum w g).
+// the f32 loop, and the matrix instructions overlap the VALU.  After every two
+// such chunks (and before any other class) the four lane groups' sums of each
+// target are added through LDS, and a_i gains (sum w g s) - t_i (sum w g).
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
@@ -523,19 +525,17 @@ __device__ __forceinline__ void mat_stage(u32x4_t* kq, float4* rec, float sx,
 }
 
 // One staged chunk of 64 sources (matrix-core class) against the wave's 64
-// targets; adds the lane's own target's sum to acc.  Reads kq quads
-// abase + 48 b (lane groups 0-2: their source's quad; group 3: the constant
-// quad 256) and rec; then reuses kq[0, 256) for the cross-lane sums.
-__device__ __forceinline__ void chunk_f32r_mat(u32x4_t* kq, const float4* rec,
+// targets, added to the lane's four-wide sums a[c] of targets 16c + col over
+// its sources 16b + 4grp + r.  Reads kq quads abase + 48 b (lane groups 0-2:
+// their source's quad; group 3: the constant quad 256) and rec.
+__device__ __forceinline__ void chunk_f32r_mat(const u32x4_t* kq,
+                                               const float4* rec,
                                                const u32x4_t (&Bt)[4],
-                                               const TargetR& t, Kahan3& acc) {
+                                               f32x4_t (&a)[4]) {
   const int lane = (int)(threadIdx.x & (kWave - 1));
   const int grp = lane >> 4, col = lane & 15;
   const int abase = grp < 3 ? 3 * col + grp : 256;
   const int astride = grp < 3 ? 48 : 0;
-  f32x4_t a[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) a[c] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
   for (int b = 0; b < 4; ++b) {
     const bf16x8_t As = __builtin_bit_cast(bf16x8_t, kq[abase + astride * b]);
@@ -560,11 +560,21 @@ __device__ __forceinline__ void chunk_f32r_mat(u32x4_t* kq, const float4* rec,
       }
     }
   }
-  wave_sync();  // every quad read before the sums overwrite them
+}
+
+// Adds the four lane groups' sums of each target through LDS (kq[0, 256),
+// free once the last chunk's quads are read), gives the lane's own target
+// (sum w g s) - t (sum w g) to the compensated sums, and clears a.
+__device__ __forceinline__ void flush_f32r_mat(u32x4_t* kq, f32x4_t (&a)[4],
+                                               const TargetR& t, Kahan3& acc) {
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  const int grp = lane >> 4, col = lane & 15;
   float4* red = (float4*)kq;
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+  for (int c = 0; c < 4; ++c) {
     red[4 * (16 * c + col) + grp] = __builtin_bit_cast(float4, a[c]);
+    a[c] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  }
   wave_sync();
   float4 f = red[4 * lane];
 #pragma unroll
@@ -578,6 +588,7 @@ __device__ __forceinline__ void chunk_f32r_mat(u32x4_t* kq, const float4* rec,
   kahan_add(acc.sx, acc.cx, fmaf(-t.x, f.w, f.x));
   kahan_add(acc.sy, acc.cy, fmaf(-t.y, f.w, f.y));
   kahan_add(acc.sz, acc.cz, fmaf(-t.z, f.w, f.z));
+  wave_sync();  // the sums are read before kq is staged again
 }
 
 // One staged chunk of 64 sources against the lane's target.  RES adds the
@@ -648,6 +659,10 @@ __device__ __forceinline__ void sweep_f32r(float4* stage, float4* res,
                                            double oz, const TargetR& t,
                                            float eps2, Kahan3& acc) {
   const int lane = (int)(threadIdx.x & (kWave - 1));
+  f32x4_t a[4];  // matrix-core class sums, cleared by each flush
+#pragma unroll
+  for (int c = 0; c < 4; ++c) a[c] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  int pend = 0;  // matrix-core chunks in a[] (wave-uniform)
   for (int64_t j0 = jlo; j0 < jhi; j0 += kWave) {
     const int64_t j = j0 + lane;
     const bool valid = j < end;
@@ -667,10 +682,17 @@ __device__ __forceinline__ void sweep_f32r(float4* stage, float4* res,
     if (all_mat) {
       mat_stage(kq, stage, sx, sy, sz, g, eps2);
       wave_sync();
-      chunk_f32r_mat(kq, stage, Bt, t, acc);
+      chunk_f32r_mat(kq, stage, Bt, a);
       wave_sync();
-      continue;
+      ++pend;
     }
+    // lanes sum 32 sources per target before the cross-lane addition; the
+    // other classes reuse kq, so they flush first
+    if (pend == 2 || (pend && !all_mat)) {
+      flush_f32r_mat(kq, a, t, acc);
+      pend = 0;
+    }
+    if (all_mat) continue;
     stage[lane] = {sx, sy, sz, g};
     const bool any_near = __ballot(valid && cl.near) != 0;  // wave-uniform
     if (any_near)
@@ -683,6 +705,7 @@ __device__ __forceinline__ void sweep_f32r(float4* stage, float4* res,
       chunk_f32r<NOMASK, false, BATCH>(stage, res, t, eps2, acc);
     wave_sync();
   }
+  if (pend) flush_f32r_mat(kq, a, t, acc);
 }
 
 // W waves per workgroup, loop shape S (see Shape).
