@@ -215,7 +215,7 @@ def classify(S, si, g):
         a = (rad + sn).astype(F)
         a2 = (a * a).astype(F)
         near = ~(a2 <= F(KN * KN) * d2)
-        d3 = (d2 * np.sqrt(d2)).astype(F)
+        d3 = (d2 * np.minimum(d2, F(1))).astype(F)   # <= D^3
         gs = (g * sn).astype(F)
         mat = ((a2 <= F(KM * KM) * d2) & (d2 >= F(2.0 ** -60)) &
                ((g * np.maximum(np.maximum(sn, rad), F(1))).astype(F) <= F(2.0 ** 100) * d3) &
