@@ -56,15 +56,17 @@ extern "C" {
 #define NBODY_PRECISION_F64_STRICT 0  /* reference operation order, IEEE sqrt and divide, no FMA: bit-identical to the f64 CPU path */
 #define NBODY_PRECISION_F64 1         /* f64, rsq + refinement, FMA; same j order */
 #define NBODY_PRECISION_F32C 2        /* f64 positions, each difference rounded once to f32, f32 pair terms, compensated sums; f64 state */
-#define NBODY_PRECISION_F32R 3        /* bodies in Hilbert order, coordinates relative to each wave's middle target split into an f32 value and an f32 residual; f32 pair terms, the residuals added for source chunks near the targets; compensated sums; <= 1e-5 relative; faster than F32C for large systems */
+#define NBODY_PRECISION_F32R 3        /* bodies in Hilbert order, coordinates relative to each wave's middle target split into an f32 value and an f32 residual; f32 pair terms, the residuals added for source chunks near the targets, r^2 of chunks far from them from bf16-split Gram products on the matrix cores; compensated sums; <= 1e-5 relative; faster than F32C for systems above about 2000 bodies */
 /* Zero softening (eps2 == 0) with two distinct bodies at one point:
  * F64_STRICT, F64 and F32C follow the reference (1/sqrt(0) * 0 gives NaN
  * for both bodies).  F32R excludes a target's own term by its zero
  * difference, so such bodies get the finite sum over their other sources. */
 /* Range of the f32 forms (F32C, F32R): r^2 and k = g*m*(r^2 + eps2)^-1.5
  * are formed in f32.
- *  - Pair distances must stay below about 1e18 (r^2 in f32 range); wider
- *    systems need NBODY_PRECISION_F64.
+ *  - Pair distances must stay below about 1e18 (r^2 in f32 range), and
+ *    beyond about 4e12 the factor r^-3 leaves f32's normal range (it
+ *    vanishes beyond about 1e15): systems wider than about 1e12 length
+ *    units need NBODY_PRECISION_F64.
  *  - Tiny softening: once g*m*eps^-3 exceeds FLT_MAX (eps below about
  *    1.4e-13 for g*m = 1), k overflows at close range.  F32C skips the
  *    target's own index.  F32R drops its own pair by the zero difference:
