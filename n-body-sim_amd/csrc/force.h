@@ -377,9 +377,9 @@ __device__ __forceinline__ float uniform_f32(float v) {  // into an SGPR
 // Source classes by the bound R + |s| <= K * D (see above): near unless it
 // holds at K = kNearK; matrix-core class (far enough for r^2 from the Gram
 // form on the matrix cores, below) when it holds at K = kMatK, D^2 >= 2^-60,
-// and the source's terms stay in range there: g max(|s|, R, 1) <= 2^100
+// and the source's terms stay in range there: |g| max(|s|, R, 1) <= 2^100
 // D^2 min(D^2, 1) <= 2^100 D^3 (so w g, w g |s| and R w g stay below 2^100
-// with w = r^-3 <= D^-3) and g |s| <= 2^120 (the staged g s).  Overflow or NaN in a test makes the
+// in magnitude with w = r^-3 <= D^-3) and |g| |s| <= 2^120 (the staged g s).  Overflow or NaN in a test makes the
 // source near (the exact path) or keeps it out of the matrix-core class.
 // The tests run in f32 with v_sqrt_f32 (about 1 ulp); its rounding moves K
 // by a few parts in 10^7.
@@ -406,9 +406,10 @@ __device__ __forceinline__ SrcClass classify_source(const TargetR& t, float sx,
   const bool near = !(a2 <= (kNearK * kNearK) * d2);
   if (!NBODY_F32R_MATRIX) return {near, false};
   const float d3 = d2 * fminf(d2, 1.f);  // <= D^3
-  const float gs = g * sn;
+  const float ga = fabsf(g);  // masses may be negative
+  const float gs = ga * sn;
   const bool mat = (a2 <= (kMatK * kMatK) * d2) && (d2 >= kMatMinD2) &&
-                   (g * fmaxf(fmaxf(sn, t.rad), 1.f) <= 0x1p100f * d3) &&
+                   (ga * fmaxf(fmaxf(sn, t.rad), 1.f) <= 0x1p100f * d3) &&
                    (gs <= 0x1p120f);
   return {near, mat};
 }

@@ -162,6 +162,25 @@ def test_f32_tight_pairs_far_from_origin(gpu, prec):
     assert e.max() <= 1e-5 and np.median(e) <= 1e-7
 
 
+@pytest.mark.parametrize("prec", [nb.Precision.F32R, nb.Precision.F32C])
+def test_f32_cluster_with_distant_ring(gpu, prec):
+    """A cluster with a central body of mass 1 and a ring of light bodies at
+    radius 20-40 (16384 bodies): each ring body's acceleration is nearly all
+    the central body's single pair term, which F32R forms in its matrix-core
+    class (r^2 from the Gram form at |s| ~ 30), so that term's own error is
+    the body's error."""
+    from test_f32r_matrix_model import cluster_with_ring
+    m, x, y, z = cluster_with_ring(8192, 8192, 22)
+    zero = np.zeros_like(m)
+    b = nb.Bodies(m, x, y, z, zero, zero, zero)
+    ref = oracle.newton_parallel(m, x, y, z, 1.0, 1e-3)
+    e = rel_err(gpu_acc(gpu, b, 1.0, 1e-3, prec), ref)
+    ring = m < 1e-6
+    print(f"{prec.name} max {e.max():.3e} med {np.median(e):.3e} "
+          f"ring max {e[ring].max():.3e}", flush=True)
+    assert e.max() <= 1e-5 and np.median(e) <= 1e-6
+
+
 @pytest.mark.parametrize("integ", INTEG)
 @pytest.mark.parametrize("n", [3, 100, 257])
 def test_strict_trajectories_bit_exact(gpu, integ, n):
@@ -585,21 +604,30 @@ def test_f32r_astronomical_units(gpu):
     """Large units: a 4096-body Plummer sphere scaled to 1e12 length units
     with g*m = 1e27 per body.  F32R's matrix-core class stages g*s per
     source, which would reach ~1e39 and overflow f32; its range guard
-    (g |s| <= 2^120) sends those chunks to the f32 loop, which forms
+    (|g| |s| <= 2^120) sends those chunks to the f32 loop, which forms
     g*m*r^-3 (~1e-9) first and stays in range.  F32R stays finite and within
-    1e-5 of the oracle, as F32C does."""
+    1e-5 of the oracle, as F32C does.  With every mass negated (the guard
+    bounds |g|), each form returns exactly the negated accelerations."""
     b = nb.plummer(4096, 37)
     L = 1e12
-    b = nb.Bodies(np.full(4096, 1e27), b.pos_x * L, b.pos_y * L, b.pos_z * L,
-                  b.vel_x, b.vel_y, b.vel_z)
     eps = 0.005 * L
-    ref = oracle.newton_parallel(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, eps)
+    got = {}
+    for sign in (1.0, -1.0):
+        bs = nb.Bodies(np.full(4096, sign * 1e27), b.pos_x * L, b.pos_y * L,
+                       b.pos_z * L, b.vel_x, b.vel_y, b.vel_z)
+        ref = oracle.newton_parallel(bs.masses, bs.pos_x, bs.pos_y, bs.pos_z,
+                                     1.0, eps)
+        for prec in (nb.Precision.F32R, nb.Precision.F32C):
+            a = gpu_acc(gpu, bs, 1.0, eps, prec)
+            assert all(np.isfinite(c).all() for c in a), (prec.name, sign)
+            e = rel_err(a, ref)
+            print(f"{prec.name} sign {sign:+.0f}: max {e.max():.3e}",
+                  flush=True)
+            assert e.max() <= 1e-5, (prec.name, sign, e.max())
+            got[prec, sign] = a
     for prec in (nb.Precision.F32R, nb.Precision.F32C):
-        got = gpu_acc(gpu, b, 1.0, eps, prec)
-        assert all(np.isfinite(a).all() for a in got), prec.name
-        e = rel_err(got, ref)
-        print(f"{prec.name}: max {e.max():.3e}", flush=True)
-        assert e.max() <= 1e-5, (prec.name, e.max())
+        for c_pos, c_neg in zip(got[prec, 1.0], got[prec, -1.0]):
+            assert np.array_equal(c_neg, -c_pos), prec.name
 
 
 def test_kernel_timing_counts_and_bits(gpu):
