@@ -297,10 +297,12 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
     s.vcom[q] = (M != 0.0) ? v[q] / M : 0.0;
   }
   for (int64_t i = 0; i < p.n; ++i) {
-    s.ext = std::max({s.ext, std::fabs(x[i] - s.com[0]),
-                      std::fabs(y[i] - s.com[1]), std::fabs(z[i] - s.com[2])});
+    s.ext3[0] = std::max(s.ext3[0], std::fabs(x[i] - s.com[0]));
+    s.ext3[1] = std::max(s.ext3[1], std::fabs(y[i] - s.com[1]));
+    s.ext3[2] = std::max(s.ext3[2], std::fabs(z[i] - s.com[2]));
     s.mmax = std::max(s.mmax, std::fabs(m[i]));
   }
+  s.ext = std::max({s.ext3[0], s.ext3[1], s.ext3[2]});
   const size_t bytes = sizeof(double) * p.n;
   double* dst[7] = {A_.m, A_.x, A_.y, A_.z, A_.vx, A_.vy, A_.vz};
   for (int q = 0; q < 7; ++q)
@@ -1288,6 +1290,40 @@ void Engine::step_graph(const std::vector<int>& ids, int64_t nsteps) {
   run_steps(ids, nsteps, true, [](int64_t) {});
 }
 
+// The box in which MODE_F32R's target order quantises a system (k_sort_keys:
+// position 0.5 + (v - o) * scale per axis, 16 bits, then the Hilbert index).
+// Width 4 * ext about the centre of mass, advanced by vcom * time.  An axis
+// along which the system is thin (extent at most ext / 8, the normal of a
+// disc) is scaled 16 times more coarsely and its centre moved off the
+// box's mid-plane, to 0.5 + 2^-7: the slab then lies inside one cell of the
+// curve at every level down to cells of 2^-6 of the box.  Centred, the
+// curve's first split would cut the disc along its mid-plane, and every
+// 64-body run would hold bodies of one half of its thickness only, over
+// twice the area.  Depends only on the system's own state (as the order
+// must, so a system orders alike alone and batched).
+static void sort_box(const double com[3], const double vcom[3],
+                     const double ext3[3], double time, SortSys& e) {
+  const double ext = std::max({ext3[0], ext3[1], ext3[2]});
+  const double inv = ext > 0.0 ? 1.0 / (4.0 * ext) : 1.0;
+  double o[3], sc[3];
+  for (int q = 0; q < 3; ++q) {
+    const double c = com[q] + vcom[q] * time;
+    if (ext > 0.0 && ext3[q] <= 0.125 * ext) {
+      sc[q] = inv * 0x1p-4;
+      o[q] = c - 0x1p-7 / sc[q];  // c maps to 0.5 + 2^-7
+    } else {
+      sc[q] = inv;
+      o[q] = c;
+    }
+  }
+  e.ox = o[0];
+  e.oy = o[1];
+  e.oz = o[2];
+  e.ix = sc[0];
+  e.iy = sc[1];
+  e.iz = sc[2];
+}
+
 // Target-ordering passes of one call (see SortPass).  Runs only in
 // MODE_F32R with the direct sum; otherwise the plan is empty.
 std::vector<Engine::SortPass> Engine::plan_sorts(const std::vector<int>& ids,
@@ -1310,10 +1346,7 @@ std::vector<Engine::SortPass> Engine::plan_sorts(const std::vector<int>& ids,
       e.n = (int32_t)s.p.n;
       e.base = p.total;
       const double time = (double)T * s.p.dt;  // the centre moves with vcom
-      e.ox = s.com[0] + s.vcom[0] * time;
-      e.oy = s.com[1] + s.vcom[1] * time;
-      e.oz = s.com[2] + s.vcom[2] * time;
-      e.inv = s.ext > 0.0 ? 1.0 / (4.0 * s.ext) : 1.0;
+      sort_box(s.com, s.vcom, s.ext3, time, e);
       tab.push_back(e);
       ++p.nsys;
       p.total += e.n;

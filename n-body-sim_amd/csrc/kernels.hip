@@ -396,9 +396,9 @@ __global__ __launch_bounds__(kWave) void k_sort_keys(const Arena* Ap,
   if (l >= t.n) return;
   const CArena* A = (const CArena*)Ap;
   const int64_t i = t.off + l;
-  const uint64_t code = hilbert48(quant16(A->x[i], t.ox, t.inv),
-                                  quant16(A->y[i], t.oy, t.inv),
-                                  quant16(A->z[i], t.oz, t.inv));
+  const uint64_t code = hilbert48(quant16(A->x[i], t.ox, t.ix),
+                                  quant16(A->y[i], t.oy, t.iy),
+                                  quant16(A->z[i], t.oz, t.iz));
   keys[t.base + l] = ((uint64_t)r << 48) | code;
   vals[t.base + l] = l;
 }

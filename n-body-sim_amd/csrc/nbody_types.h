@@ -158,8 +158,8 @@ struct Arena {         // device pointers, passed by value to every kernel
 struct SortSys {
   int64_t off;         // arena offset
   int32_t n, base;     // bodies; first slot in the pass's key arrays
-  double ox, oy, oz;   // quantisation origin (the system's moving centre)
-  double inv;          // 1 / box width
+  double ox, oy, oz;   // quantisation origin: position 0.5 of each axis
+  double ix, iy, iz;   // per-axis scale: 1 / the box's width along the axis
 };
 
 // One integrator stage applied on its own to the acceleration arrays (ax,

@@ -27,17 +27,35 @@ U = 2.0 ** -24
 F = np.float32
 
 
-def hilbert_order(x, y, z, m):
-    """The device's target order: quant16 around the centre of mass in a box
-    of width 4 * extent, hilbert48, stable sort."""
+def sort_box(x, y, z, m, thin_axes=True):
+    """Engine's sort_box (engine.cpp): per axis, the origin that maps to 0.5
+    and the scale.  Width 4 * extent about the centre of mass; an axis whose
+    extent is at most extent / 8 is scaled 16 times more coarsely and its
+    centre placed at 0.5 + 2^-7."""
     com = [np.sum(m * v) / m.sum() for v in (x, y, z)]
-    ext = max(np.abs(v - c).max() for v, c in zip((x, y, z), com))
-    inv = 1.0 / (4.0 * ext)
+    e3 = [np.abs(v - c).max() for v, c in zip((x, y, z), com)]
+    ext = max(e3)
+    inv = 1.0 / (4.0 * ext) if ext > 0 else 1.0
+    o, sc = [], []
+    for c, e in zip(com, e3):
+        if thin_axes and ext > 0 and e <= 0.125 * ext:
+            sc.append(inv * 2.0 ** -4)
+            o.append(c - 2.0 ** -7 / sc[-1])
+        else:
+            sc.append(inv)
+            o.append(c)
+    return o, sc
 
-    def q(v, o):
-        return (np.clip((v - o) * inv + 0.5, 0.0, 0.99999) * 65536.0).astype(np.int64)
 
-    qx, qy, qz = q(x, com[0]), q(y, com[1]), q(z, com[2])
+def hilbert_order(x, y, z, m, thin_axes=True):
+    """The device's target order: quant16 in the box of sort_box, hilbert48,
+    stable sort (thin_axes=False: every axis scaled alike and centred)."""
+    o, sc = sort_box(x, y, z, m, thin_axes)
+
+    def q(v, a):
+        return (np.clip((v - o[a]) * sc[a] + 0.5, 0.0, 0.99999) * 65536.0).astype(np.int64)
+
+    qx, qy, qz = q(x, 0), q(y, 1), q(z, 2)
     keys = [hilbert48(int(a), int(b), int(c)) for a, b, c in zip(qx, qy, qz)]
     return np.argsort(np.array(keys, dtype=np.uint64), kind="stable")
 
