@@ -56,7 +56,7 @@ extern "C" {
 #define NBODY_PRECISION_F64_STRICT 0  /* reference operation order, IEEE sqrt and divide, no FMA: bit-identical to the f64 CPU path */
 #define NBODY_PRECISION_F64 1         /* f64, rsq + refinement, FMA; same j order */
 #define NBODY_PRECISION_F32C 2        /* f64 positions, each difference rounded once to f32, f32 pair terms, compensated sums; f64 state */
-#define NBODY_PRECISION_F32R 3        /* bodies in Hilbert order, coordinates relative to each wave's middle target split into an f32 value and an f32 residual; f32 pair terms, the residuals added for source chunks near the targets, r^2 of chunks far from them from bf16-split Gram products on the matrix cores; compensated sums; <= 1e-5 relative; faster than F32C for systems above about 2000 bodies */
+#define NBODY_PRECISION_F32R 3        /* bodies in Hilbert order, coordinates relative to each wave's middle target split into an f32 value and an f32 residual; f32 pair terms, the residuals added for source chunks near the targets, r^2 of chunks far from them from bf16-split Gram products on the matrix cores; compensated sums; <= 1e-5 relative; faster than F32C for systems above about 1500 bodies */
 /* Zero softening (eps2 == 0) with two distinct bodies at one point:
  * F64_STRICT, F64 and F32C follow the reference (1/sqrt(0) * 0 gives NaN
  * for both bodies).  F32R excludes a target's own term by its zero
