@@ -406,3 +406,97 @@ def test_sort_box_leaves_round_systems_alone():
     m, x, y, z = plummer(2048, 8)
     assert np.array_equal(hilbert_order(x, y, z, m, True),
                           hilbert_order(x, y, z, m, False))
+
+
+def boundary_pair_system(seed, margin=1e-4):
+    """A 128-body system whose 64 massless targets each feel exactly one
+    pair term taken by the matrix-core class at its boundary.
+
+    Targets: 64 massless bodies in a cube of side 2 (random offset).
+    Sources: one body of mass 1 and 63 massless copies at the same point P,
+    placed along a random direction (into the positive octant of the
+    targets' centre) so that (R + |s|) / D = 4 (1 - margin)
+    for the targets' wave (R, s, D about the wave's origin, its middle
+    target in the device's order).  The copies fill the source chunk, which
+    the class takes only whole; their terms are exact zeros.  Returns
+    (m, x, y, z) in body order."""
+    rng = np.random.default_rng(seed)
+    T = rng.uniform(-1, 1, (64, 3)) + rng.uniform(-5, 5, 3)
+    m = np.concatenate([np.zeros(64), [1.0], np.zeros(63)])
+
+    def bodies(P):
+        p = np.vstack([T, np.repeat(P[None], 64, 0)])
+        return p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy()
+
+    def ratio(P, o):
+        t = T - o
+        lo, hi = t.min(0), t.max(0)
+        s = P - o
+        d = np.linalg.norm(np.maximum(np.maximum(lo - s, s - hi), 0))
+        a = np.sqrt((t * t).sum(1)).max() + np.linalg.norm(s)
+        return a / d if d > 0 else np.inf
+
+    c = T.mean(0)
+
+    def place(j):
+        """P on the boundary for target j as the origin, and the body at
+        the targets' middle slot in the order P gives."""
+        lo_l, hi_l = 1e-3, 1e4         # the ratio falls as P moves out
+        for _ in range(200):
+            mid = np.sqrt(lo_l * hi_l)
+            if ratio(c + mid * u, T[j]) > KM * (1 - margin):
+                lo_l = mid
+            else:
+                hi_l = mid
+        P = c + hi_l * u
+        perm = hilbert_order(*bodies(P), m)
+        tile = perm[:64] if perm[:64].max() < 64 else perm[64:]
+        assert tile.max() < 64, "targets and sources share a tile"
+        return P, int(tile[32])
+
+    # a direction into the positive octant: P ends above every target on
+    # every axis.  P carries all the mass, so it is the ordering box's centre
+    # and the targets fall in another top-level octant of the curve: the two
+    # groups never share a 64-body run.  The origin depends on the order,
+    # which depends on P: look for a target that is the middle one when P
+    # sits on its own boundary, following the order's answer first, and take
+    # another direction if there is none.
+    while True:
+        u = np.abs(rng.standard_normal(3))
+        u /= np.linalg.norm(u)
+        if u.min() < 0.45:
+            continue
+        tried = set()
+        j = int(hilbert_order(*bodies(c + 20 * u), m)[32])
+        while True:
+            tried.add(j)
+            P, mid = place(j)
+            if mid == j or len(tried) == 64:
+                break
+            j = mid if mid not in tried else min(set(range(64)) - tried)
+        if mid == j:
+            break
+    assert (P > T.max(0)).all()
+    return (m,) + bodies(P)
+
+
+def test_boundary_pair_systems_lie_in_the_class():
+    """The model's classify, with the device's order and origin, puts every
+    source of the boundary systems' source chunk in the matrix-core class
+    for the targets' wave, at a ratio within 1e-3 of K = 4."""
+    for seed in range(16):
+        m, x, y, z = boundary_pair_system(seed)
+        perm = hilbert_order(x, y, z, m)
+        P = np.stack([x, y, z], 1)[perm]
+        t0 = 0 if perm[:64].max() < 64 else 64
+        c0 = 64 - t0
+        S = (P - P[t0 + 32]).astype(F)
+        si = S[t0:t0 + 64]
+        g = m[perm].astype(F)
+        near, mat = classify(S, si, g)
+        assert mat[c0:c0 + 64].all() and not near[c0:c0 + 64].any(), seed
+        rad = np.sqrt((si.astype(np.float64) ** 2).sum(1)).max()
+        s = S[c0].astype(np.float64)
+        lo, hi = si.min(0), si.max(0)
+        d = np.linalg.norm(np.maximum(np.maximum(lo - s, s - hi), 0))
+        assert (rad + np.linalg.norm(s)) / d > KM * (1 - 1e-3), seed

@@ -630,6 +630,38 @@ def test_f32r_astronomical_units(gpu):
             assert np.array_equal(c_neg, -c_pos), prec.name
 
 
+def test_f32r_matrix_class_pairs_at_the_boundary(gpu):
+    """One pair term per target, taken by F32R's matrix-core class at its
+    boundary (R + |s| = 4 D within 1e-4): 64 massless targets against one
+    unit mass and 63 massless copies of it (test_f32r_matrix_model's
+    boundary_pair_system), 48 geometries, unsoftened and with softening 0.5.
+    Each target's acceleration is that single term, so its relative error is
+    the class's per-pair error on the hardware (r^2 from the bf16-split Gram
+    products, r^-3, and the (sum w g s) - t (sum w g) form), with nothing
+    averaged away.  Held to the 1e-5 contract; the measured worst case is
+    printed in units of 2^-24, beside F32C's on the same pairs (one rounding
+    of each f64 difference, no matrix cores)."""
+    from test_f32r_matrix_model import boundary_pair_system
+    errs = {nb.Precision.F32R: [], nb.Precision.F32C: []}
+    for seed in range(48):
+        m, x, y, z = boundary_pair_system(seed)
+        eps = 0.0 if seed % 2 == 0 else 0.5
+        zero = np.zeros_like(m)
+        b = nb.Bodies(m, x, y, z, zero, zero, zero)
+        ref = oracle.newton_parallel(m, x, y, z, 1.0, eps)
+        for prec in errs:
+            got = gpu_acc(gpu, b, 1.0, eps, prec)
+            errs[prec].append(rel_err([g[:64] for g in got],
+                                      [r[:64] for r in ref]))
+    u = 2.0 ** -24
+    for prec, es in errs.items():
+        e = np.concatenate(es)
+        print(f"boundary pairs {prec.name}: max {e.max():.3e} "
+              f"({e.max() / u:.1f} units of 2^-24), median {np.median(e):.3e} "
+              f"({np.median(e) / u:.1f})", flush=True)
+        assert np.isfinite(e).all() and e.max() <= 1e-5, prec.name
+
+
 def test_kernel_timing_counts_and_bits(gpu):
     """Launch timing brackets every force and combine launch with events:
     five velocity-Verlet steps are five force launches (one combine each when
