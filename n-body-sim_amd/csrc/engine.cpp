@@ -281,7 +281,7 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
   for (int m = 0; m < 4; ++m) s.variant[m] = tuning_variant(m, p.n);
   // centre of mass and its velocity, f64 in index order (the centre of the
   // MODE_F32R ordering box, advanced by vcom * elapsed time), and the
-  // largest coordinate distance from it (the box size)
+  // largest distance from it along each axis (the box size)
   double M = 0, c[3] = {0, 0, 0}, v[3] = {0, 0, 0};
   for (int64_t i = 0; i < p.n; ++i) {
     M += m[i];
@@ -302,7 +302,6 @@ int Engine::add(const SystemParams& p, const double* m, const double* x,
     s.ext3[2] = std::max(s.ext3[2], std::fabs(z[i] - s.com[2]));
     s.mmax = std::max(s.mmax, std::fabs(m[i]));
   }
-  s.ext = std::max({s.ext3[0], s.ext3[1], s.ext3[2]});
   const size_t bytes = sizeof(double) * p.n;
   double* dst[7] = {A_.m, A_.x, A_.y, A_.z, A_.vx, A_.vy, A_.vz};
   for (int q = 0; q < 7; ++q)

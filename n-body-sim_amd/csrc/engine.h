@@ -218,7 +218,7 @@ class Engine {
     double com[3] = {0, 0, 0};  // centre of mass at upload (sort box centre,
     double vcom[3] = {0, 0, 0};  // advanced with the centre-of-mass velocity)
     double ext3[3] = {0, 0, 0};  // largest distance from com along each axis
-    double ext = 0.0;           // the largest of them
+                                 // (the sort box's size)
     double mmax = 0.0;          // largest |m| (bounds the f32 self term)
     int64_t sorted_at = -1;     // step count of the last body ordering
     // staged integration (integrate_stage): the stage that must come next
