@@ -461,7 +461,7 @@ def boundary_pair_system(seed, margin=1e-4):
     # which depends on P: look for a target that is the middle one when P
     # sits on its own boundary, following the order's answer first, and take
     # another direction if there is none.
-    while True:
+    for _ in range(1000):
         u = np.abs(rng.standard_normal(3))
         u /= np.linalg.norm(u)
         if u.min() < 0.45:
@@ -476,6 +476,8 @@ def boundary_pair_system(seed, margin=1e-4):
             j = mid if mid not in tried else min(set(range(64)) - tried)
         if mid == j:
             break
+    else:
+        raise RuntimeError(f"no boundary placement found for seed {seed}")
     assert (P > T.max(0)).all()
     return (m,) + bodies(P)
 
