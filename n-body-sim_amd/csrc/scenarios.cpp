@@ -116,13 +116,16 @@ bool ok_ptrs(const double* a, const double* b, const double* c,
   return a && b && c && d && e && f && g;
 }
 
+// v > 0 and finite (false for NaN).
+bool positive(double v) { return v > 0.0 && std::isfinite(v); }
+
 }  // namespace
 
 extern "C" int nbody_scenario_plummer(int64_t n, uint64_t seed, double a,
                                       double total_mass, double g, double* m,
                                       double* x, double* y, double* z,
                                       double* vx, double* vy, double* vz) {
-  if (n <= 0 || !(a > 0) || !(total_mass > 0) || !(g > 0) ||
+  if (n <= 0 || !positive(a) || !positive(total_mass) || !positive(g) ||
       !ok_ptrs(m, x, y, z, vx, vy, vz))
     return NBODY_E_INVALID;
   Rng rng(seed);
@@ -143,8 +146,10 @@ extern "C" int nbody_scenario_star_system(int64_t n, uint64_t seed,
                                           double g, double* m, double* x,
                                           double* y, double* z, double* vx,
                                           double* vy, double* vz) {
-  if (n <= 0 || !(central_mass > 0) || !(m_lo > 0) || !(m_hi >= m_lo) ||
-      !(r_lo > 0) || !(r_hi >= r_lo) || !(max_incl_deg >= 0) || !(g > 0) ||
+  if (n <= 0 || !positive(central_mass) || !positive(m_lo) ||
+      !(m_hi >= m_lo) || !std::isfinite(m_hi) || !positive(r_lo) ||
+      !(r_hi >= r_lo) || !std::isfinite(r_hi) || !(max_incl_deg >= 0) ||
+      !std::isfinite(max_incl_deg) || !positive(g) ||
       !ok_ptrs(m, x, y, z, vx, vy, vz))
     return NBODY_E_INVALID;
   Rng rng(seed);
@@ -181,7 +186,7 @@ extern "C" int nbody_scenario_colliding(int64_t n, uint64_t seed,
                                         double g, double* m, double* x,
                                         double* y, double* z, double* vx,
                                         double* vy, double* vz) {
-  if (n < 2 || !(g > 0) || !std::isfinite(separation) ||
+  if (n < 2 || !positive(g) || !std::isfinite(separation) ||
       !std::isfinite(rel_speed) || !ok_ptrs(m, x, y, z, vx, vy, vz))
     return NBODY_E_INVALID;
   Rng rng(seed);
@@ -216,8 +221,9 @@ extern "C" int nbody_scenario_disc(int64_t n, uint64_t seed, double radius,
                                    double softening, double g, double* m,
                                    double* x, double* y, double* z,
                                    double* vx, double* vy, double* vz) {
-  if (n <= 0 || !(radius > 0) || !(body_mass > 0) || !(density_power > 0) ||
-      !(softening >= 0) || !(g > 0) || !std::isfinite(radius) ||
+  if (n <= 0 || !positive(radius) || !positive(body_mass) ||
+      !positive(density_power) || !(softening >= 0) ||
+      !std::isfinite(softening) || !positive(g) ||
       !ok_ptrs(m, x, y, z, vx, vy, vz))
     return NBODY_E_INVALID;
   Rng rng(seed);
@@ -234,8 +240,9 @@ extern "C" int nbody_scenario_disc(int64_t n, uint64_t seed, double radius,
   for (int64_t k = 0; k < n; ++k) {
     const int64_t i = order[k];
     const double menc = (double)(k + 1) * body_mass;
-    const double v = std::sqrt(g * menc * r[i] /
-                               (r[i] * r[i] + softening * softening));
+    const double d2 = r[i] * r[i] + softening * softening;
+    // a body drawn at the centre with no softening rests there (0 / 0)
+    const double v = d2 > 0.0 ? std::sqrt(g * menc * r[i] / d2) : 0.0;
     const double disp = v * 0.1;
     m[k] = body_mass;
     x[k] = r[i] * std::cos(th[i]);

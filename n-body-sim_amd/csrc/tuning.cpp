@@ -12,6 +12,8 @@
 #include <dlfcn.h>
 
 #include <algorithm>
+#include <cerrno>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -51,6 +53,29 @@ void init() {
     tuning_load(env);
   } else {
     tuning_load((library_dir() + "/tuning/gfx950.txt").c_str());
+  }
+}
+
+// The whitespace-separated integers of `s` after its first word `word`
+// (at most 3); returns how many were read, or -1 when the word differs, a
+// token is not an integer or one leaves the int range (strtol, not sscanf's
+// %d, whose overflow is undefined).
+int parse_entry(const char* s, const char* word, int* v) {
+  const size_t wl = std::strlen(word);
+  if (std::strncmp(s, word, wl) != 0 || (s[wl] != ' ' && s[wl] != '\t'))
+    return -1;
+  const char* p = s + wl;
+  int k = 0;
+  for (;;) {
+    while (*p == ' ' || *p == '\t') ++p;
+    if (*p == '\0' || *p == '\n' || *p == '\r' || *p == '#') return k;
+    if (k == 3) return -1;
+    char* end = nullptr;
+    errno = 0;
+    const long x = std::strtol(p, &end, 10);
+    if (end == p || errno == ERANGE || x < INT_MIN || x > INT_MAX) return -1;
+    v[k++] = (int)x;
+    p = end;
   }
 }
 
@@ -109,16 +134,24 @@ bool tuning_load(const char* path) {
   std::memcpy(tmp, g_split, sizeof(tmp));
   std::memcpy(tv, g_variant, sizeof(tv));
   while (std::fgets(line, sizeof(line), f)) {
+    // a line longer than the buffer is dropped whole
+    const bool whole = std::strchr(line, '\n') || std::feof(f);
+    if (!whole) {
+      int c;
+      while ((c = std::fgetc(f)) != EOF && c != '\n') {
+      }
+      continue;
+    }
     if (line[0] == '#') continue;
-    int b = -1, k = 0, m = -1;
-    if (std::sscanf(line, "split %d %d", &b, &k) == 2 && b >= 0 &&
-        b < kTuningBuckets && k >= 1) {
-      tmp[b] = std::min(k, kMaxSplit);
+    int v[3];
+    if (parse_entry(line, "split", v) == 2 && v[0] >= 0 &&
+        v[0] < kTuningBuckets && v[1] >= 1) {
+      tmp[v[0]] = std::min(v[1], kMaxSplit);
       ++loaded;
-    } else if (std::sscanf(line, "variant %d %d %d", &m, &b, &k) == 3 &&
-               m > MODE_F64_STRICT && m <= MODE_F32R && b >= 0 &&
-               b < kTuningBuckets && k >= 0 && k < kForceVariants) {
-      tv[m][b] = k;
+    } else if (parse_entry(line, "variant", v) == 3 && v[0] > MODE_F64_STRICT &&
+               v[0] <= MODE_F32R && v[1] >= 0 && v[1] < kTuningBuckets &&
+               v[2] >= 0 && v[2] < kForceVariants) {
+      tv[v[0]][v[1]] = v[2];
       ++loaded;
     }
   }

@@ -70,6 +70,38 @@ def test_null_and_invalid_arguments():
                                           *ptrs) == nb.E_INVALID   # m_hi < m_lo
 
 
+def test_scenarios_reject_non_finite_parameters():
+    """Every generator parameter must be finite: an infinite scale, mass,
+    radius, angle or G would otherwise give non-finite bodies under
+    NBODY_OK."""
+    lib = nb.load_library()
+    arrs = [np.empty(4) for _ in range(7)]
+    ptrs = [nb._p(a) for a in arrs]
+    inf, nan = float("inf"), float("nan")
+    plummer = [1.0, 1.0, 1.0]                        # a, total mass, g
+    for k in range(3):
+        for bad in (inf, nan):
+            args = list(plummer)
+            args[k] = bad
+            assert lib.nbody_scenario_plummer(4, 1, *args, *ptrs) == nb.E_INVALID
+    star = [1.0, 1e-7, 1e-4, 0.5, 10.0, 5.0, 1.0]    # central, m_lo, m_hi, r_lo, r_hi, incl, g
+    for k in range(7):
+        args = list(star)
+        args[k] = inf
+        assert lib.nbody_scenario_star_system(4, 1, *args, *ptrs) == nb.E_INVALID, k
+    assert lib.nbody_scenario_colliding(4, 1, 10.0, 0.5, inf, *ptrs) == nb.E_INVALID
+    disc = [1.0, 1e-3, 0.5, 0.0, 1.0]                # radius, mass, power, softening, g
+    for k in range(5):
+        args = list(disc)
+        args[k] = inf
+        assert lib.nbody_scenario_disc(4, 1, *args, *ptrs) == nb.E_INVALID, k
+    # the valid parameter sets themselves succeed, with finite bodies
+    assert lib.nbody_scenario_plummer(4, 1, *plummer, *ptrs) == nb.OK
+    assert lib.nbody_scenario_star_system(4, 1, *star, *ptrs) == nb.OK
+    assert lib.nbody_scenario_disc(4, 1, *disc, *ptrs) == nb.OK
+    assert all(np.isfinite(a).all() for a in arrs)
+
+
 def test_plummer_properties():
     b = nb.plummer(20000, 7)
     m = b.masses
