@@ -268,7 +268,9 @@ int nbody_energy_momentum(nbody_ctx *ctx, nbody_bodies *b, double g,
 
 /* ---- scenario generators (no reference counterpart; the star system
  * follows gui/generators.py:11-98).  Deterministic for a seed (own
- * xoshiro256** stream).  Output arrays hold n values each. */
+ * xoshiro256** stream).  Output arrays hold n values each.  Every
+ * parameter must be finite (NBODY_E_INVALID otherwise), so a generator
+ * that returns NBODY_OK wrote finite bodies. */
 /* Plummer sphere: scale radius a, total mass, G; equal masses, virial
  * velocities, centre-of-mass frame. */
 int nbody_scenario_plummer(int64_t n, uint64_t seed, double a,
