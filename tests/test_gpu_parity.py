@@ -128,17 +128,31 @@ def test_config2_size_f64_and_f32c(gpu):
     assert e32r.max() <= 1e-5 and np.median(e32r) <= 1e-7
 
 
-@pytest.mark.parametrize("scenario", ["star", "colliding", "disc"])
+def tilted(b, ax, az):
+    """The bodies rotated by ax about x, then az about z (radians)."""
+    cx, sx, cz, sz = np.cos(ax), np.sin(ax), np.cos(az), np.sin(az)
+    R = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]]) @ \
+        np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    p = R @ np.stack([b.pos_x, b.pos_y, b.pos_z])
+    v = R @ np.stack([b.vel_x, b.vel_y, b.vel_z])
+    return nb.Bodies(b.masses, *[a.copy() for a in p], *[a.copy() for a in v])
+
+
+@pytest.mark.parametrize("scenario", ["star", "colliding", "disc", "tilted_disc"])
 def test_f32_scenarios_within_1e5(gpu, scenario):
     """The f32 forms on the other workloads' geometries: a central mass with
-    a thin population of light bodies, two separated spheres, a thin disc."""
+    a thin population of light bodies, two separated spheres, a thin disc in
+    the x-y plane (F32R orders it in a box with a coarse, off-centre z axis)
+    and the same disc tilted off every axis (ordered in the centred cube)."""
     if scenario == "star":
         b, eps = nb.star_system(6000, 11, m_lo=1e-7, m_hi=1e-4, r_lo=0.5,
                                 r_hi=10.0, max_incl_deg=5.0), 1e-3
     elif scenario == "colliding":
         b, eps = nb.colliding(8192, 12, separation=10.0, rel_speed=0.5), 0.002
-    else:
+    elif scenario == "disc":
         b, eps = nb.disc_galaxy(8192, 13), 0.05
+    else:
+        b, eps = tilted(nb.disc_galaxy(8192, 13), 0.7, 0.45), 0.05
     ref = oracle.newton_parallel(b.masses, b.pos_x, b.pos_y, b.pos_z, 1.0, eps)
     for prec in (nb.Precision.F32R, nb.Precision.F32C):
         e = rel_err(gpu_acc(gpu, b, 1.0, eps, prec), ref)
