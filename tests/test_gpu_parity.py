@@ -91,6 +91,33 @@ def test_strict_accelerations_bit_exact(gpu, n):
         assert np.array_equal(got[q], ref[q]), f"component {q}"
 
 
+@pytest.mark.parametrize("integ", INTEG)
+def test_single_body_every_precision_bit_exact(gpu, integ):
+    """One body: its acceleration is exactly zero in every arithmetic form
+    (it has no source but itself), so every form's trajectory is the
+    reference's free drift bit for bit, softened or not."""
+    b = nb.Bodies(np.array([2.5]), np.array([0.3]), np.array([-1.7]),
+                  np.array([1e3]), np.array([0.11]), np.array([-0.07]),
+                  np.array([2.9]))
+    dt, k = 1e-3, 9
+    for prec in nb.Precision:
+        for eps in (0.0, 0.01):
+            acc = gpu_acc(gpu, b, 1.0, eps, prec)
+            assert all(np.array_equal(a, [0.0]) for a in acc), (prec.name, eps)
+            ref = oracle.integrate(int(integ), b.masses, b.pos_x, b.pos_y,
+                                   b.pos_z, b.vel_x, b.vel_y, b.vel_z, 1.0,
+                                   eps, dt, k)
+            db = nb.DeviceBodies(gpu, b)
+            grav = nb.GpuNewtonParallelGravity(1.0, eps * eps, prec)
+            integ.gpu_create(dt).step(gpu, db, grav, k)
+            out = db.download()
+            db.close()
+            for q, f in enumerate(("pos_x", "pos_y", "pos_z", "vel_x",
+                                   "vel_y", "vel_z")):
+                assert np.array_equal(getattr(out, f), ref[q]), \
+                    (prec.name, eps, f)
+
+
 @pytest.mark.parametrize("name", sorted(EXAMPLES))
 def test_examples_all_precisions(gpu, name):
     b, g, dt, eps = example(name)
