@@ -118,6 +118,29 @@ def test_single_body_every_precision_bit_exact(gpu, integ):
                     (prec.name, eps, f)
 
 
+def test_f32r_input_order_does_not_change_the_bits(gpu):
+    """F32R orders targets and sources by the Hilbert index of their own
+    positions, so a permuted input gives the same accelerations, permuted, bit
+    for bit (no two of these bodies share a quantisation cell).  F64 and F32C
+    sum sources in input order, so for them the permutation moves only the
+    last bits, within each form's tolerance."""
+    b = nb.plummer(4096, 77)
+    perm = np.random.default_rng(5).permutation(4096)
+    bp = nb.Bodies(*[getattr(b, f)[perm].copy() for f in
+                     ("masses", "pos_x", "pos_y", "pos_z", "vel_x", "vel_y",
+                      "vel_z")])
+    for prec, tol in ((nb.Precision.F32R, 0.0), (nb.Precision.F32C, 1e-5),
+                      (nb.Precision.F64, 1e-12)):
+        a = gpu_acc(gpu, b, 1.0, 0.01, prec)
+        ap = gpu_acc(gpu, bp, 1.0, 0.01, prec)
+        if tol == 0.0:
+            for c in range(3):
+                assert np.array_equal(ap[c], a[c][perm]), (prec.name, c)
+        else:
+            e = rel_err([x for x in ap], [x[perm] for x in a])
+            assert e.max() <= 2 * tol, (prec.name, e.max())
+
+
 @pytest.mark.parametrize("name", sorted(EXAMPLES))
 def test_examples_all_precisions(gpu, name):
     b, g, dt, eps = example(name)
